@@ -1,0 +1,195 @@
+/*
+ * oceansar_hip.h -- C ABI of liboceansar_hip.so, the MI355X (gfx950) kernels
+ * behind the OCEANSAR raw-data generation path.
+ *
+ * The reference path is pure Python (numpy / numba); its "interfaces" are
+ * Python call sites. Each entry point below replaces one of them:
+ *
+ *   osar_surface_synth   OceanSurface.t setter
+ *                        (oceansar/surfaces/ocean.py:418-526)
+ *   osar_facet_bragg     RCSRomeiser97.rcs -> rcs_numba -> _rcs_numba, plus the
+ *                        sqrt(RCS) and NRCS sums of the driver
+ *                        (oceansar/nrcs/romeiser97.py:95-107,251-393;
+ *                         oceansar/radarsim/sar_raw_nompi.py:432-467)
+ *   osar_echo_accumulate the per-pulse body of sar_raw from the surface fields
+ *                        to the range lines: geometry, Bragg scene, beam,
+ *                        channel phases and chan_profile_numba /
+ *                        profile_integrator
+ *                        (oceansar/radarsim/sar_raw_nompi.py:349-535;
+ *                         oceansar/radarsim/range_profile.py:22-49,83-116)
+ *   osar_echo_deposit    chan_profile_numba alone (range_profile.py:83-116):
+ *                        deposit given complex facet values at given ranges
+ *
+ * Conventions
+ *   - Every pointer is a caller-owned device buffer; the library never frees
+ *     caller memory. Workspace is owned by the context.
+ *   - Return 0 on success, a negative OSAR_E* code on error; the message of
+ *     the last error on the calling thread is osar_last_error().
+ *   - Calls are asynchronous on the given HIP stream (a hipStream_t passed as
+ *     void*, NULL = default stream).
+ *   - One context per host thread or stream; contexts share no mutable state.
+ */
+#ifndef OCEANSAR_HIP_H
+#define OCEANSAR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OSAR_OK 0
+#define OSAR_EINVAL (-1)        /* bad argument */
+#define OSAR_ENOMEM (-2)        /* out of device memory */
+#define OSAR_ELAUNCH (-3)       /* kernel launch / runtime failure */
+#define OSAR_EUNSUPPORTED (-4)  /* configuration not supported */
+
+#define OSAR_MAX_CH 8
+
+typedef struct osar_ctx osar_ctx;
+
+/* float2 / complex64 as laid out in memory: {re, im} */
+typedef struct osar_c32 { float re, im; } osar_c32;
+
+/* ---- context ---------------------------------------------------------- */
+int osar_ctx_create(int device, osar_ctx** out);
+int osar_ctx_destroy(osar_ctx* ctx);
+const char* osar_last_error(void);
+/* Library version: major*10000 + minor*100 + patch. */
+int osar_version(void);
+/* sizeof of the descriptor structs (0 echo, 1 bragg, 2 surface), so that
+   bindings can check their layout; -1 for an unknown id. */
+int64_t osar_struct_size(int which);
+
+/* ---- echo accumulation ------------------------------------------------ */
+/* Flags of osar_echo_desc.flags */
+#define OSAR_ECHO_BRAGG 1      /* add the two Bragg components */
+#define OSAR_ECHO_XTI 2        /* apply the cross-track phase (any b_xti != 0) */
+
+typedef struct osar_echo_desc {
+  /* sizes */
+  int32_t nx, ny;           /* facets per row, rows */
+  int32_t n_pulses_total;   /* pulses (rows) of the raw cube */
+  int32_t pulse0, npulses;  /* window of pulses processed by this call */
+  int32_t nbins;            /* bins per range line (rg_samp) */
+  int32_t n_ch, n_pol;      /* channels (<= OSAR_MAX_CH), polarisations (1 or 2) */
+  int32_t K;                /* surface snapshot cadence in pulses */
+  int32_t snap0;            /* snapshot index stored in slot 0 of dx/dy/dz */
+  int32_t n_snap_slots;     /* slots in dx/dy/dz */
+  int32_t rcs0;             /* RCS refresh index stored in slot 0 of amp */
+  int32_t n_rcs_slots;      /* slots in amp */
+  int32_t flags;            /* OSAR_ECHO_* */
+  /* launch shape (0 = library default) */
+  int32_t pulses_per_block; /* 1, 2, 4 or 8 */
+  int32_t rows_per_block;
+  int32_t window_bins;      /* LDS range-line window per (pol, ch, pulse) */
+  int32_t reserved;
+  /* FP64 scalars */
+  double t_step, t_span, v_ground, sr0, gr0, k0;
+  double min_sr, binsize;   /* bin position x = (sr - min_sr) / binsize */
+  double ltx_over_l, lrx_over_l;  /* antenna lengths over wavelength */
+  double sr_col_min, sr_col_max;  /* range of sr_col */
+  double resid_max;         /* bound on |-cos_inc Dz + sin_inc Dx + sin_az Dy| */
+  double b_ati[OSAR_MAX_CH], b_xti[OSAR_MAX_CH];
+  /* per-column FP64 tables [nx] */
+  const double* sr_col;     /* slant range minus its minimum */
+  const double* sin_inc;
+  const double* cos_inc;
+  const double* dinc_col;   /* inc - inc_angle */
+  const double* cap_rate;   /* 2 k0 t_step c_b; Bragg phase is cap_rate * (p + 1) */
+  /* per-row FP64 table [ny] */
+  const double* y_row;
+  /* surface snapshots, float32 [n_snap_slots][ny][nx] */
+  const float* dx;
+  const float* dy;
+  const float* dz;
+  /* Bragg amplitudes sqrt(RCS), float32 [n_rcs_slots][n_pol][2][ny][nx];
+     component 0 pairs with scat_m, component 1 with scat_p */
+  const float* amp;
+  const int32_t* rcs_slot;  /* [n_pulses_total] RCS refresh index in force */
+  /* scatterer series, extended: s_ext[i] = s[i mod N], i < N + n_pulses_total */
+  const osar_c32* scat_m;
+  const osar_c32* scat_p;
+  const int32_t* ind0_m;    /* [ny][nx], 0 <= ind0 < N */
+  const int32_t* ind0_p;
+  const float* sinc_tab;    /* [161] range response table */
+  /* output: accumulated, complex64 [n_pol][n_ch][n_pulses_total][nbins] */
+  osar_c32* raw;
+} osar_echo_desc;
+
+int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void* stream);
+
+/* Zero-fill n complex64 values (a kernel, so it may be captured in a graph). */
+int osar_fill_zero_c32(osar_ctx* ctx, osar_c32* p, int64_t n, void* stream);
+
+/* chan_profile_numba: out[d - 4 + j] += tab[sb + 20 j] * scene[i] for every
+   facet i, with d = floor((srg - min_sr)/binsize) and
+   sb = rint(20 (1 - frac)); taps outside [0, nbins) are dropped.
+   srg float64 [n], scene complex64 [n], out complex64 [nbins]. */
+int osar_echo_deposit(osar_ctx* ctx, const double* srg, const osar_c32* scene,
+                      int64_t n, double binsize, double min_sr,
+                      const float* sinc_tab, int32_t n_sinc_samples,
+                      int32_t sinc_ovs, osar_c32* out, int32_t nbins,
+                      void* stream);
+
+/* ---- Bragg facet kernel ----------------------------------------------- */
+typedef struct osar_bragg_desc {
+  int32_t nx, ny;
+  int32_t pol;              /* 0 = vv, 1 = hh, 2 = DP (hh then vv) */
+  int32_t spread;           /* 0 = elfouhaily, 1 = cos2s */
+  int32_t n_spec;           /* spectrum table length */
+  int32_t pulse;            /* refresh pulse (sets the azimuth of every row) */
+  double k0, d, wind_dir, wind_U, wind_fetch;
+  double dx, dy;            /* facet size (RCS = NRCS * dx * dy) */
+  double t_step, t_span, v_ground, gr0;
+  const double* inc_col;    /* [nx] incidence angle */
+  const double* y_row;      /* [ny] */
+  const double* spec_k;     /* [n_spec] ascending */
+  const double* spec_S;     /* [n_spec] */
+  const float* diffx;       /* [ny][nx] */
+  const float* diffy;
+  float* amp;               /* out: [n_pol][2][ny][nx] sqrt(RCS) */
+  double* rcs_sum;          /* out: [n_pol] sum of RCS over facets and components
+                               (accumulated; zero it first) */
+} osar_bragg_desc;
+
+int osar_facet_bragg(osar_ctx* ctx, const osar_bragg_desc* d, void* stream);
+int osar_fill_zero_f64(osar_ctx* ctx, double* p, int64_t n, void* stream);
+
+/* ---- surface synthesis ------------------------------------------------ */
+/* Output field bits of osar_surface_desc.fields */
+#define OSAR_F_DX 1
+#define OSAR_F_DY 2
+#define OSAR_F_DZ 4
+#define OSAR_F_DIFFX 8
+#define OSAR_F_DIFFY 16
+#define OSAR_F_DIFFXX 32
+#define OSAR_F_DIFFYY 64
+#define OSAR_F_DIFFXY 128
+
+typedef struct osar_surface_desc {
+  int32_t nx, ny;           /* powers of two, 16 .. 4096 */
+  int32_t fields;           /* OSAR_F_* mask of outputs to write */
+  int32_t choppy;           /* 1: choppy slope / curvature corrections */
+  int32_t n_times;          /* snapshots computed by this call */
+  int32_t reserved;
+  double depth;             /* water depth, <= 0 for deep water */
+  double current_x, current_y;
+  double kx_res, ky_res;    /* float32 grid steps (good-k threshold) */
+  const float* kx;          /* [nx] float32 wave numbers (2 pi fftfreq) */
+  const float* ky;          /* [ny] */
+  const osar_c32* coefs;    /* [ny][nx] complex64 wave coefficients */
+  const double* times;      /* [n_times] snapshot times */
+  /* outputs: float32 [n_times][ny][nx] for every requested field, NULL otherwise */
+  float* out[8];            /* indexed by bit position: dx, dy, dz, diffx, diffy,
+                               diffxx, diffyy, diffxy */
+} osar_surface_desc;
+
+/* Workspace bytes osar_surface_synth needs for this descriptor. */
+int64_t osar_surface_workspace_bytes(const osar_surface_desc* d);
+int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OCEANSAR_HIP_H */

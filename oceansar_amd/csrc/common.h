@@ -1,0 +1,72 @@
+// Shared helpers of liboceansar_hip.so (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+
+#include "oceansar_hip.h"
+
+struct osar_ctx {
+  int device;
+  int n_cu;
+  size_t max_lds;             // bytes of LDS one workgroup may allocate
+  void* work;                 // synthesis workspace (grows on demand)
+  size_t work_bytes;
+};
+
+// Thread-local last error message (osar_last_error).
+void osar_set_error(const char* fmt, ...);
+
+#define OSAR_REQUIRE(cond, code, ...)      \
+  do {                                     \
+    if (!(cond)) {                         \
+      osar_set_error(__VA_ARGS__);         \
+      return (code);                       \
+    }                                      \
+  } while (0)
+
+#define OSAR_HIP(call)                                                   \
+  do {                                                                   \
+    hipError_t e_ = (call);                                              \
+    if (e_ != hipSuccess) {                                              \
+      osar_set_error("%s failed: %s", #call, hipGetErrorString(e_));     \
+      return e_ == hipErrorOutOfMemory ? OSAR_ENOMEM : OSAR_ELAUNCH;     \
+    }                                                                    \
+  } while (0)
+
+// Check the launch that was just issued.
+#define OSAR_LAUNCHED(name)                                                      \
+  do {                                                                           \
+    hipError_t e_ = hipGetLastError();                                           \
+    if (e_ != hipSuccess) {                                                      \
+      osar_set_error("launch of %s failed: %s", name, hipGetErrorString(e_));    \
+      return OSAR_ELAUNCH;                                                       \
+    }                                                                            \
+  } while (0)
+
+// Pointer must be set (the ABI takes raw device pointers).
+#define OSAR_PTR(p) OSAR_REQUIRE((p) != nullptr, OSAR_EINVAL, "null pointer: %s", #p)
+
+namespace osar {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double kInvTwoPi = 0.15915494309189533576888376337251;
+
+// exp(i * phase) for an FP64 phase of any size: the phase is reduced to
+// revolutions in FP64 (exact subtraction of the nearest integer), and only the
+// reduced fraction in [-1/2, 1/2] goes through FP32 sin/cos.
+__device__ __forceinline__ float2 cis_fp64(double phase) {
+  double r = phase * kInvTwoPi;
+  double f = r - rint(r);
+  float s, c;
+  sincospif(2.0f * (float)f, &s, &c);
+  return make_float2(c, s);
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+}  // namespace osar

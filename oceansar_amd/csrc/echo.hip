@@ -1,0 +1,361 @@
+// Echo accumulation: per-pulse facet geometry, Bragg scene, beam, channel
+// phases and the 8-tap range deposit, fused in one kernel.
+//
+// Replaces the body of the sar_raw pulse loop (oceansar/radarsim/
+// sar_raw_nompi.py:349-535) from the surface fields to the range lines,
+// including chan_profile_numba / profile_integrator
+// (oceansar/radarsim/range_profile.py:22-49,83-116).
+//
+// Work decomposition: a workgroup owns P consecutive pulses and R rows of
+// facets. Each wavefront lane is one (facet, pulse) pair: 64/P consecutive
+// columns of one row times P pulses, so the fields of a facet are loaded once
+// for P pulses and every lane of an instruction deposits into a different
+// (pulse) line or, within a line, into a tap rotated by the facet's lane
+// index, which keeps LDS atomics of neighbouring facets off the same address.
+// Range lines of the P pulses (all channels and polarisations) live in an LDS
+// window of W bins; they are added to the raw cube with one atomic pass per
+// workgroup. Taps that fall outside the window (but inside the line) go
+// straight to global memory; taps outside [0, nbins) are dropped as in the
+// reference.
+//
+// Precision: slant range, bin position, phases and the lerp of the
+// displacement fields are FP64 in the reference's operation order (FP
+// contraction off), so bin and sub-bin indices match an FP64 evaluation; the
+// phase is reduced to revolutions in FP64 before FP32 sin/cos; scene values,
+// channel factors and the accumulation are FP32.
+#include "common.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kTaps = 8;
+constexpr int kOvs = 20;
+constexpr int kTab = kTaps * kOvs + 1;  // 161
+
+__device__ __forceinline__ double np_sinc(double x) {
+  // numpy.sinc: sin(pi x)/(pi x) with x = 0 replaced by 1e-20
+  double y = M_PI * (x == 0.0 ? 1.0e-20 : x);
+  return sin(y) / y;
+}
+
+struct Geo {
+  double srs;   // slant range of the facet minus min(sr)
+  double dinc;  // wave_dinc (for the cross-track phase)
+};
+
+// sr_surface and wave_dinc (sar_raw_nompi.py:366-369), in the reference's order.
+__device__ __forceinline__ Geo facet_range(double sr, double cosi, double sini, double mig,
+                                           double sin_az, double Dx, double Dy, double Dz,
+                                           double sr0) {
+#pragma clang fp contract(off)
+  Geo g;
+  double a = sr - cosi * Dz;
+  a = a + mig;
+  a = a + Dx * sini;
+  a = a + Dy * sin_az;
+  g.srs = a;
+  g.dinc = (Dz * sini + Dx * sini) / sr0;
+  return g;
+}
+
+__device__ __forceinline__ double lerp_field(const float* f, size_t o0, size_t step, int rk,
+                                             double w) {
+#pragma clang fp contract(off)
+  double a = (double)f[o0];
+  if (rk == 0) return a;
+  double b = (double)f[o0 + step];
+  return (1. - w) * a + w * b;
+}
+
+template <int P>
+__global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_desc d, int W,
+                                                                   int R) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int npol = d.n_pol, nch = d.n_ch;
+  const int nlines = npol * nch * P;
+  double* geo = (double*)lds_raw;                        // [P][R][2] sin_az, mig
+  float2* F = (float2*)(geo + 2 * P * R);                // [P][nch][R]
+  float* acc = (float*)(F + P * nch * R);                // [nlines][2][W]
+  float* tab = acc + (size_t)nlines * 2 * W;             // [kTab]
+  int* wlo = (int*)(tab + 164);                          // [P]
+
+  const int tid = threadIdx.x;
+  const int pstart = d.pulse0 + blockIdx.x * P;
+  const int pend = d.pulse0 + d.npulses;
+  const int y0 = blockIdx.y * R;
+  const int rows = min(R, d.ny - y0);
+
+  for (int i = tid; i < nlines * 2 * W; i += kThreads) acc[i] = 0.f;
+  for (int i = tid; i < kTab; i += kThreads) tab[i] = d.sinc_tab[i];
+  for (int i = tid; i < P * R; i += kThreads) {
+    const int pi = i / R, r = i - pi * R;
+    const int p = pstart + pi;
+    double sin_az = 0., mig = 0.;
+    double beam = 0.;
+    if (p < pend && r < rows) {
+      const double t_now = p * d.t_step;
+      const double az_now = (t_now - d.t_span / 2.) * d.v_ground;
+      const double az = d.y_row[y0 + r] - az_now;
+      sin_az = az / d.sr0;
+      mig = (az / 2) * sin_az;
+      beam = np_sinc(d.ltx_over_l * sin_az) * np_sinc(d.lrx_over_l * sin_az);
+    }
+    geo[2 * i] = sin_az;
+    geo[2 * i + 1] = mig;
+    for (int ch = 0; ch < nch; ++ch) {
+      const double ph = -(d.k0 * d.b_ati[ch]) * sin_az;
+      double s, c;
+      sincos(ph, &s, &c);
+      F[(pi * nch + ch) * R + r] = make_float2((float)(beam * c), (float)(beam * s));
+    }
+  }
+  __syncthreads();
+  if (tid < P) {
+    double mmin = 1e300, mmax = -1e300;
+    for (int r = 0; r < rows; ++r) {
+      const double m = geo[2 * (tid * R + r) + 1];
+      mmin = fmin(mmin, m);
+      mmax = fmax(mmax, m);
+    }
+    const double lo = (d.sr_col_min + mmin - d.resid_max - d.min_sr) / d.binsize;
+    wlo[tid] = max((int)floor(lo) - 5, 0);
+  }
+  __syncthreads();
+
+  const int lane = tid & 63;
+  const int pi = lane & (P - 1);
+  const int fsub = lane / P;
+  constexpr int FPW = 64 / P;
+  constexpr int FPB = (kThreads / 64) * FPW;
+  const int wave = tid >> 6;
+  const int p = pstart + pi;
+  const bool active = p < pend;
+  const int pc = active ? p : pstart;
+  const int m = pc / d.K;
+  const int rk = pc - m * d.K;
+  const double w = (double)rk / (double)d.K;
+  const size_t plane = (size_t)d.nx * d.ny;
+  const size_t snap_off = (size_t)(m - d.snap0) * plane;
+  const int rslot = (d.flags & OSAR_ECHO_BRAGG) ? d.rcs_slot[pc] - d.rcs0 : 0;
+  const int wb = wlo[pi];
+  const int wend = min(wb + W, d.nbins);
+  const double two_k0 = 2 * d.k0;
+  const bool bragg = (d.flags & OSAR_ECHO_BRAGG) != 0;
+  const bool xti = (d.flags & OSAR_ECHO_XTI) != 0;
+
+  for (int r = 0; r < rows; ++r) {
+    const int yrow = y0 + r;
+    const double sin_az = geo[2 * (pi * R + r)];
+    const double mig = geo[2 * (pi * R + r) + 1];
+    for (int x = wave * FPW + fsub; x < d.nx; x += FPB) {
+      if (!active) continue;
+      const size_t idx = (size_t)yrow * d.nx + x;
+      const size_t so = snap_off + idx;
+      const double Dx = lerp_field(d.dx, so, plane, rk, w);
+      const double Dy = lerp_field(d.dy, so, plane, rk, w);
+      const double Dz = lerp_field(d.dz, so, plane, rk, w);
+      const Geo g = facet_range(d.sr_col[x], d.cos_inc[x], d.sin_inc[x], mig, sin_az, Dx, Dy,
+                                Dz, d.sr0);
+      const double xb = (g.srs - d.min_sr) / d.binsize;
+      if (!(fabs(xb) < 1.0e9)) continue;  // non-finite or absurd range: no valid tap
+      const double fl = floor(xb);
+      const int dbin = (int)fl;
+      const int sb = min(max((int)rint(20.0 * (1.0 - (xb - fl))), 0), kOvs);
+
+      float2 scene[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+      if (bragg && rslot >= 0 && rslot < d.n_rcs_slots) {
+        const double surf = -two_k0 * g.srs;
+        const double cap = d.cap_rate[x] * (double)(p + 1);
+        const float2 e0 = osar::cis_fp64(surf - cap);
+        const float2 e1 = osar::cis_fp64(surf + cap);
+        const osar_c32 sm = d.scat_m[d.ind0_m[idx] + p];
+        const osar_c32 sp = d.scat_p[d.ind0_p[idx] + p];
+        const float2 b0 = osar::cmul(e0, make_float2(sm.re, sm.im));
+        const float2 b1 = osar::cmul(e1, make_float2(sp.re, sp.im));
+        for (int pol = 0; pol < npol; ++pol) {
+          const size_t ao = ((size_t)(rslot * npol + pol) * 2) * plane + idx;
+          const float a0 = d.amp[ao];
+          const float a1 = d.amp[ao + plane];
+          scene[pol] = make_float2(a0 * b0.x + a1 * b1.x, a0 * b0.y + a1 * b1.y);
+        }
+      }
+      const bool inwin = (dbin - 4 >= wb) && (dbin + 4 <= wend);
+      const int base = dbin - 4 - wb;
+      for (int ch = 0; ch < nch; ++ch) {
+        float2 f = F[(pi * nch + ch) * R + r];
+        if (xti && d.b_xti[ch] != 0.0) {
+          const double tot = d.dinc_col[x] + g.dinc;
+          f = osar::cmul(f, osar::cis_fp64(-(d.k0 * d.b_xti[ch]) * tot));
+        }
+        for (int pol = 0; pol < npol; ++pol) {
+          const float2 v = osar::cmul(scene[pol], f);
+          float* lre = acc + (size_t)(((pol * nch + ch) * P + pi) * 2) * W;
+          float* lim = lre + W;
+          if (inwin) {
+#pragma unroll
+            for (int j = 0; j < kTaps; ++j) {
+              const int jj = (j + fsub) & (kTaps - 1);
+              const float t = tab[sb + kOvs * jj];
+              atomicAdd(&lre[base + jj], t * v.x);
+              atomicAdd(&lim[base + jj], t * v.y);
+            }
+          } else {
+            for (int j = 0; j < kTaps; ++j) {
+              const int b = dbin - 4 + j;
+              if (b < 0 || b >= d.nbins) continue;
+              const float t = tab[sb + kOvs * j];
+              if (b >= wb && b < wend) {
+                atomicAdd(&lre[b - wb], t * v.x);
+                atomicAdd(&lim[b - wb], t * v.y);
+              } else {
+                osar_c32* o = d.raw + ((size_t)(pol * nch + ch) * d.n_pulses_total + p) *
+                                          d.nbins + b;
+                unsafeAtomicAdd(&o->re, t * v.x);
+                unsafeAtomicAdd(&o->im, t * v.y);
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // Add the LDS windows to the raw cube.
+  for (int i = tid; i < nlines * W; i += kThreads) {
+    const int line = i / W, b = i - line * W;
+    const int lp = line % P;
+    const int pc2 = line / P;  // pol * nch + ch
+    const int pp = pstart + lp;
+    const int gb = wlo[lp] + b;
+    if (pp >= pend || gb >= d.nbins) continue;
+    const float re = acc[(size_t)line * 2 * W + b];
+    const float im = acc[(size_t)line * 2 * W + W + b];
+    if (re == 0.f && im == 0.f) continue;
+    osar_c32* o = d.raw + ((size_t)pc2 * d.n_pulses_total + pp) * d.nbins + gb;
+    unsafeAtomicAdd(&o->re, re);
+    unsafeAtomicAdd(&o->im, im);
+  }
+}
+
+template <int P>
+int launch_accumulate(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipStream_t s) {
+  const int nlines = d->n_pol * d->n_ch * P;
+  const size_t lds = sizeof(double) * 2 * P * R + sizeof(float2) * P * d->n_ch * R +
+                     sizeof(float) * (size_t)nlines * 2 * W + sizeof(float) * 164 +
+                     sizeof(int) * P;
+  OSAR_REQUIRE(lds <= ctx->max_lds, OSAR_EUNSUPPORTED,
+               "echo window needs %zu B of LDS (limit %zu); lower window_bins or "
+               "pulses_per_block", lds, ctx->max_lds);
+  OSAR_HIP(hipFuncSetAttribute((const void*)echo_accumulate_kernel<P>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  dim3 grid((d->npulses + P - 1) / P, (d->ny + R - 1) / R);
+  hipLaunchKernelGGL(echo_accumulate_kernel<P>, grid, dim3(kThreads), lds, s, *d, W, R);
+  OSAR_LAUNCHED("echo_accumulate_kernel");
+  return OSAR_OK;
+}
+
+// chan_profile_numba: one thread per facet, taps added with global atomics.
+__global__ void echo_deposit_kernel(const double* srg, const osar_c32* scene, int64_t n,
+                                    double binsize, double min_sr, const float* tab, int nsamp,
+                                    int ovs, osar_c32* out, int nbins) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xb = (srg[i] - min_sr) / binsize;
+  if (!(fabs(xb) < 1.0e9)) return;
+  const double fl = floor(xb);
+  const int dbin = (int)fl;
+  const int sb = min(max((int)rint((double)ovs * (1.0 - (xb - fl))), 0), ovs);
+  const osar_c32 v = scene[i];
+  const int b0 = dbin - nsamp / 2;
+  for (int j = 0; j < nsamp; ++j) {
+    const int b = b0 + j;
+    if (b < 0 || b >= nbins) continue;
+    const float t = tab[sb + j * ovs];
+    unsafeAtomicAdd(&out[b].re, t * v.re);
+    unsafeAtomicAdd(&out[b].im, t * v.im);
+  }
+}
+
+}  // namespace
+
+extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void* stream) {
+  OSAR_PTR(ctx);
+  OSAR_PTR(d);
+  OSAR_REQUIRE(d->nx > 0 && d->ny > 0 && d->nbins > 0, OSAR_EINVAL, "empty scene");
+  OSAR_REQUIRE(d->n_ch >= 1 && d->n_ch <= OSAR_MAX_CH, OSAR_EINVAL, "n_ch must be 1..%d",
+               OSAR_MAX_CH);
+  OSAR_REQUIRE(d->n_pol == 1 || d->n_pol == 2, OSAR_EINVAL, "n_pol must be 1 or 2");
+  OSAR_REQUIRE(d->K >= 1, OSAR_EINVAL, "K must be >= 1");
+  OSAR_REQUIRE(d->pulse0 >= 0 && d->npulses >= 0 &&
+                   d->pulse0 + d->npulses <= d->n_pulses_total,
+               OSAR_EINVAL, "pulse window [%d, %d) outside [0, %d)", d->pulse0,
+               d->pulse0 + d->npulses, d->n_pulses_total);
+  if (d->npulses == 0) return OSAR_OK;
+  // Every snapshot the window touches must be resident.
+  const int m_first = d->pulse0 / d->K;
+  const int p_last = d->pulse0 + d->npulses - 1;
+  const int m_last = p_last / d->K + ((p_last % d->K) ? 1 : 0);
+  OSAR_REQUIRE(m_first >= d->snap0 && m_last < d->snap0 + d->n_snap_slots, OSAR_EINVAL,
+               "snapshots %d..%d not in slots [%d, %d)", m_first, m_last, d->snap0,
+               d->snap0 + d->n_snap_slots);
+  OSAR_PTR(d->sr_col);
+  OSAR_PTR(d->sin_inc);
+  OSAR_PTR(d->cos_inc);
+  OSAR_PTR(d->y_row);
+  OSAR_PTR(d->dx);
+  OSAR_PTR(d->dy);
+  OSAR_PTR(d->dz);
+  OSAR_PTR(d->sinc_tab);
+  OSAR_PTR(d->raw);
+  if (d->flags & OSAR_ECHO_BRAGG) {
+    OSAR_PTR(d->amp);
+    OSAR_PTR(d->rcs_slot);
+    OSAR_PTR(d->scat_m);
+    OSAR_PTR(d->scat_p);
+    OSAR_PTR(d->ind0_m);
+    OSAR_PTR(d->ind0_p);
+    OSAR_PTR(d->cap_rate);
+  }
+  if (d->flags & OSAR_ECHO_XTI) OSAR_PTR(d->dinc_col);
+  int P = d->pulses_per_block ? d->pulses_per_block : 8;
+  const int R = d->rows_per_block ? d->rows_per_block : 64;
+  int W = d->window_bins ? d->window_bins : 512;
+  OSAR_REQUIRE(R >= 1 && R <= 1024 && W >= 8 && W <= 65536, OSAR_EINVAL,
+               "bad launch shape R=%d W=%d", R, W);
+  hipStream_t s = (hipStream_t)stream;
+  OSAR_HIP(hipSetDevice(ctx->device));
+  switch (P) {
+    case 1: return launch_accumulate<1>(ctx, d, W, R, s);
+    case 2: return launch_accumulate<2>(ctx, d, W, R, s);
+    case 4: return launch_accumulate<4>(ctx, d, W, R, s);
+    case 8: return launch_accumulate<8>(ctx, d, W, R, s);
+    default: break;
+  }
+  osar_set_error("pulses_per_block must be 1, 2, 4 or 8 (got %d)", P);
+  return OSAR_EINVAL;
+}
+
+extern "C" int osar_echo_deposit(osar_ctx* ctx, const double* srg, const osar_c32* scene,
+                                 int64_t n, double binsize, double min_sr,
+                                 const float* sinc_tab, int32_t n_sinc_samples,
+                                 int32_t sinc_ovs, osar_c32* out, int32_t nbins,
+                                 void* stream) {
+  OSAR_PTR(ctx);
+  OSAR_REQUIRE(n >= 0 && nbins > 0, OSAR_EINVAL, "bad sizes");
+  OSAR_REQUIRE(n_sinc_samples > 0 && n_sinc_samples <= 64 && sinc_ovs > 0, OSAR_EINVAL,
+               "bad sinc table shape");
+  OSAR_REQUIRE(binsize > 0, OSAR_EINVAL, "binsize must be positive");
+  if (n == 0) return OSAR_OK;
+  OSAR_PTR(srg);
+  OSAR_PTR(scene);
+  OSAR_PTR(sinc_tab);
+  OSAR_PTR(out);
+  OSAR_HIP(hipSetDevice(ctx->device));
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(echo_deposit_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)stream, srg, scene, n, binsize, min_sr, sinc_tab,
+                     n_sinc_samples, sinc_ovs, out, nbins);
+  OSAR_LAUNCHED("echo_deposit_kernel");
+  return OSAR_OK;
+}

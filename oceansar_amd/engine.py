@@ -1,0 +1,189 @@
+"""Device-resident scene and the GPU form of the sar_raw pulse loop.
+
+DeviceScene uploads the host set-up of a SarScene once (geometry tables,
+closure series, spectrum table, range-response table, wave coefficients) and
+drives the three kernels of liboceansar_hip.so: surface synthesis
+(osar_surface_synth), Bragg facets (osar_facet_bragg) and echo accumulation
+(osar_echo_accumulate). torch is used only to own device buffers.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+
+POL_CODE = {'vv': 0, 'hh': 1, 'DP': 2}
+
+
+def _t(a, dtype, device):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
+
+
+class LaunchShape:
+    """Echo-accumulator launch shape: pulses per workgroup, rows per workgroup, LDS window."""
+
+    def __init__(self, pulses_per_block=0, rows_per_block=0, window_bins=0):
+        self.pulses_per_block = pulses_per_block
+        self.rows_per_block = rows_per_block
+        self.window_bins = window_bins
+
+    def __repr__(self):
+        return 'LaunchShape(P=%d, R=%d, W=%d)' % (self.pulses_per_block, self.rows_per_block,
+                                                 self.window_bins)
+
+
+class DeviceScene:
+    def __init__(self, sc, device=0, lds_budget=80 * 1024):
+        import torch
+        self.sc = sc
+        self.ctx = _lib.context(device)
+        self.dev = torch.device('cuda', device)
+        dev = self.dev
+        f64, f32, i32, c64 = torch.float64, torch.float32, torch.int32, torch.complex64
+        self.sr_col = _t(sc.sr, f64, dev)
+        self.sin_inc = _t(sc.sin_inc, f64, dev)
+        self.cos_inc = _t(sc.cos_inc, f64, dev)
+        self.inc_col = _t(sc.inc, f64, dev)
+        self.dinc_col = _t(sc.inc - sc.inc_angle, f64, dev)
+        self.y_row = _t(sc.y64, f64, dev)
+        self.sinc = _t(sc.sinc_vec, f32, dev)
+        self.npol = len(sc.pols)
+        self.n_pulses = sc.az_steps
+        if sc.bragg_enable:
+            self.cap_rate = _t(sc.cap_rate, f64, dev)
+            self.scat_m = _t(self._extend(sc.scat_m), c64, dev)
+            self.scat_p = _t(self._extend(sc.scat_p), c64, dev)
+            self.ind0_m = _t(np.mod(sc.scat_m.ind0, sc.scat_m.N), i32, dev)
+            self.ind0_p = _t(np.mod(sc.scat_p.ind0, sc.scat_p.N), i32, dev)
+            self.rcs_slot = _t(sc.rcs_slot, i32, dev)
+            self.spec_k = _t(sc.bragg.spec_k, f64, dev)
+            self.spec_S = _t(sc.bragg.spec_S, f64, dev)
+        surf = sc.surface
+        self.coefs = _t(surf.wave_coefs, c64, dev)
+        self.kx = _t(surf.kx_s, f32, dev)
+        self.ky = _t(surf.ky_s, f32, dev)
+        self.resid_max = self._resid_bound()
+        self.shape = self.default_shape(lds_budget)
+
+    # ------------------------------------------------------------------
+    def _extend(self, rs):
+        """s_ext[i] = s[i mod N] for i < N + n_pulses, so the kernel indexes ind0 + p."""
+        n = rs.N + self.n_pulses
+        return rs.s[np.arange(n) % rs.N].astype(np.complex64)
+
+    def _resid_bound(self):
+        """Window margin for |residual range|: 6 sigma of |Dz|+|Dx|+|Dy| from the spectrum.
+
+        Only sizes the LDS window; facets beyond it take the global-memory path.
+        """
+        c = self.sc.surface.wave_coefs.astype(np.complex128)
+        n2 = float(self.sc.Nx * self.sc.Ny)
+        rms = math.sqrt(float(np.sum(np.abs(c) ** 2)) / 2.) / n2
+        cur = float(np.hypot(*self.sc.surface.current)) * self.n_pulses * self.sc.t_step
+        return 6. * 3. * rms + cur + 0.5
+
+    def default_shape(self, lds_budget=80 * 1024, rows=64):
+        sc = self.sc
+        R = min(rows, sc.Ny)
+        az_max = max(abs(sc.az_now(0)), abs(sc.az_now(sc.az_steps))) + sc.surface.Ly
+        mig_spread = R * float(sc.surface.dy) * az_max / sc.sr0
+        span = (sc.sr.max() - sc.sr.min() + mig_spread + 2 * self.resid_max) / sc.binsize
+        W = int(math.ceil(span)) + 8 + 4
+        W = min(((W + 31) // 32) * 32, ((sc.rg_samp + 31) // 32) * 32)
+        lines = self.npol * sc.num_ch
+        for P in (8, 4, 2, 1):
+            lds = 16 * P * R + 8 * P * sc.num_ch * R + 8 * lines * P * W + 700
+            if lds <= lds_budget:
+                return LaunchShape(P, R, W)
+        return LaunchShape(1, R, W)
+
+    # ------------------------------------------------------------------
+    def new_raw(self):
+        import torch
+        sc = self.sc
+        raw = torch.empty((self.npol, sc.num_ch, self.n_pulses, sc.rg_samp),
+                          dtype=torch.complex64, device=self.dev)
+        self.zero_c32(raw)
+        return raw
+
+    def zero_c32(self, t):
+        _lib.check(self.ctx.lib.osar_fill_zero_c32(self.ctx.h, _lib.ptr(t), t.numel(),
+                                                   _lib.stream_ptr()))
+
+    def zero_f64(self, t):
+        _lib.check(self.ctx.lib.osar_fill_zero_f64(self.ctx.h, _lib.ptr(t), t.numel(),
+                                                   _lib.stream_ptr()))
+
+    def bragg(self, diffx, diffy, pulse, amp_out, rcs_sum_out=None):
+        """sqrt(RCS) of every facet at refresh pulse `pulse` into amp_out [npol][2][Ny][Nx]."""
+        sc, b = self.sc, self.sc.bragg
+        d = _lib.BraggDesc()
+        d.nx, d.ny = sc.Nx, sc.Ny
+        d.pol = POL_CODE[sc.pol] if sc.pol in POL_CODE else 0
+        d.spread = b.spread_id
+        d.n_spec = len(b.spec_k)
+        d.pulse = int(pulse)
+        d.k0, d.d, d.wind_dir, d.wind_U, d.wind_fetch = b.k0, b.d, b.wind_dir, b.wind_U, b.wind_fetch
+        d.dx, d.dy = float(sc.surface.dx), float(sc.surface.dy)
+        d.t_step, d.t_span, d.v_ground, d.gr0 = sc.t_step, sc.t_span, sc.v_ground, sc.gr0
+        d.inc_col, d.y_row = _lib.ptr(self.inc_col), _lib.ptr(self.y_row)
+        d.spec_k, d.spec_S = _lib.ptr(self.spec_k), _lib.ptr(self.spec_S)
+        d.diffx, d.diffy = _lib.ptr(diffx), _lib.ptr(diffy)
+        d.amp = _lib.ptr(amp_out)
+        d.rcs_sum = _lib.ptr(rcs_sum_out)
+        _lib.check(self.ctx.lib.osar_facet_bragg(self.ctx.h, ctypes.byref(d), _lib.stream_ptr()))
+
+    def echo_desc(self, dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape=None):
+        sc = self.sc
+        shape = shape or self.shape
+        d = _lib.EchoDesc()
+        d.nx, d.ny = sc.Nx, sc.Ny
+        d.n_pulses_total = self.n_pulses
+        d.pulse0, d.npulses = int(pulse0), int(npulses)
+        d.nbins = sc.rg_samp
+        d.n_ch, d.n_pol = sc.num_ch, self.npol
+        d.K = sc.K
+        d.snap0 = int(snap0)
+        d.n_snap_slots = int(dx.shape[0])
+        d.rcs0 = int(rcs0)
+        d.n_rcs_slots = int(amp.shape[0]) if amp is not None else 0
+        flags = 0
+        if sc.bragg_enable:
+            flags |= _lib.ECHO_BRAGG
+        if np.any(sc.b_xti != 0):
+            flags |= _lib.ECHO_XTI
+        d.flags = flags
+        d.pulses_per_block = shape.pulses_per_block
+        d.rows_per_block = shape.rows_per_block
+        d.window_bins = shape.window_bins
+        d.t_step, d.t_span, d.v_ground = sc.t_step, sc.t_span, sc.v_ground
+        d.sr0, d.gr0, d.k0 = sc.sr0, sc.gr0, sc.k0
+        d.min_sr, d.binsize = sc.min_sr, sc.sr_res / sc.over_fs
+        d.ltx_over_l = sc.ant_l_tx / sc.l0
+        d.lrx_over_l = sc.ant_l_rx / sc.l0
+        d.sr_col_min, d.sr_col_max = float(sc.sr.min()), float(sc.sr.max())
+        d.resid_max = self.resid_max
+        for i in range(sc.num_ch):
+            d.b_ati[i] = sc.b_ati[i]
+            d.b_xti[i] = sc.b_xti[i]
+        d.sr_col, d.sin_inc, d.cos_inc = (_lib.ptr(self.sr_col), _lib.ptr(self.sin_inc),
+                                          _lib.ptr(self.cos_inc))
+        d.dinc_col, d.y_row = _lib.ptr(self.dinc_col), _lib.ptr(self.y_row)
+        d.dx, d.dy, d.dz = _lib.ptr(dx), _lib.ptr(dy), _lib.ptr(dz)
+        d.sinc_tab = _lib.ptr(self.sinc)
+        d.raw = _lib.ptr(raw)
+        if sc.bragg_enable:
+            d.cap_rate = _lib.ptr(self.cap_rate)
+            d.amp = _lib.ptr(amp)
+            d.rcs_slot = _lib.ptr(self.rcs_slot)
+            d.scat_m, d.scat_p = _lib.ptr(self.scat_m), _lib.ptr(self.scat_p)
+            d.ind0_m, d.ind0_p = _lib.ptr(self.ind0_m), _lib.ptr(self.ind0_p)
+        return d
+
+    def accumulate(self, dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape=None):
+        """Add pulses [pulse0, pulse0+npulses) of every channel/pol into raw (device)."""
+        d = self.echo_desc(dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape)
+        _lib.check(self.ctx.lib.osar_echo_accumulate(self.ctx.h, ctypes.byref(d),
+                                                     _lib.stream_ptr()))

@@ -1,0 +1,28 @@
+"""FP64 CPU oracle of the OCEANSAR raw-data generation path.
+
+TEST INFRASTRUCTURE ONLY. This package is the checker, never the product:
+only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import it, call it, or load its C library. Nothing in
+``oceansar_amd`` imports it, and the product path raises if the HIP library
+is missing instead of falling back to anything here.
+
+Each function restates the reference algorithm from its source text
+(file:line citations are to the reference repository). The random draws and
+the host set-up (coefficients, geometry tables, closure series, Bragg
+spectrum table, range-response table) are inputs, shared with the GPU path.
+
+Contents and how each is pinned (see tests/test_oracle_*.py):
+  surface.synth       a2  numpy complex128 ifft2 restatement; pinned by a
+                          single-mode closed form (elevation, displacements,
+                          slopes, curvatures, choppy forms) and Parseval.
+  bragg.rcs_romeiser97 a9 pinned by the flat-surface SPM closed form, slope
+                          symmetry, the bad-facet mask and spread half plane.
+  ka.field_spa        a10 pinned by a paraboloid with known curvatures and by
+                          the flat-facet normal-incidence limit.
+  echo.profile_integrator / chan_profile  a13/a14  pinned by brute-force
+                          sinc evaluation, a point target, edge dropping,
+                          linearity and a one-bin shift.
+  sar.simulate        a1/a6/a11/a12  per-pulse loop with integer cadence;
+                          pinned by superposition of point targets and by
+                          the C restatement (c/oracle_echo.c) agreeing with it.
+"""
