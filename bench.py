@@ -1,0 +1,150 @@
+"""Throughput of the OCEANSAR raw-data generation hot path on one MI355X.
+
+A step is one full raw-data simulation of the workload: surface synthesis
+(FP64 FFTs) every K pulses, Bragg facet RCS at every refresh, and the fused
+echo accumulation of every facet into every pulse's range line for every
+channel. Inputs (wave coefficients, geometry tables, closure series) are
+resident in HBM before the timed region; the raw cube stays on the device.
+
+Metric: facet-pulse contributions per second (one facet's echo deposited into
+one pulse's range line for one channel and polarisation).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 3]
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 (vector, packed) peak
+HBM_PEAK_GBS = 8000.0
+
+
+def flop_per_contribution(n_ch, n_pol):
+    """Algorithmic FP32 FLOP of one contribution: 38 per (facet, pulse, channel, pol)
+    (8 taps x real*complex MAC = 32, channel phasor 6) plus 35 per (facet, pulse, pol)
+    (residual range, binning, the two Bragg terms, surface phasor, beam) shared by
+    the channels."""
+    return 38. + 35. / n_ch
+
+
+def cpu_baseline(workload, sample_pulses, threads):
+    """The C restatement of the reference loop (plus the numpy FP64 synthesis and
+    Bragg RCS it needs) timed on host cores over a bounded pulse sample."""
+    os.environ['OMP_NUM_THREADS'] = str(threads)
+    from oceansar_amd import configs
+    import oracle.c_oracle as co
+    import oracle.sar as osar
+    sc = configs.load_scene(configs.baseline(workload), tempfile.mkdtemp())
+    p0 = sc.az_steps // 2
+    p1 = p0 + sample_pulses
+    t = time.perf_counter()
+    co.simulate_c(sc, p0, p1, snaps=osar.SnapshotCache(sc))
+    dt = time.perf_counter() - t
+    w = sc.Nx * sc.Ny * (p1 - p0) * sc.num_ch * len(sc.pols)
+    return {'value': w / dt, 'unit': 'contributions/s', 'cores': threads, 'kind': 'port',
+            'sample': 'workload %d, pulses [%d, %d) of %d (%d snapshots, %d Bragg refreshes), '
+                      '%.1f s' % (workload, p0, p1, sc.az_steps, (p1 - 1) // sc.K - p0 // sc.K + 2,
+                                  len(set(sc.rcs_slot[p0:p1])), dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--workload', type=int, default=3)
+    ap.add_argument('--pulses', type=int, default=None, help='shorter scene (testing only)')
+    ap.add_argument('--chunk', type=int, default=32, help='snapshot intervals per launch group')
+    ap.add_argument('--cpu-sample', type=int, default=32)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only: this build has no multi-GPU path '
+              '(pulse sharding across GPUs is a design note, not implemented)', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from oceansar_amd import configs
+    from oceansar_amd.engine import DeviceScene
+
+    over = {} if args.pulses is None else {'n_pulses': args.pulses}
+    sc = configs.load_scene(configs.baseline(args.workload, **over), tempfile.mkdtemp())
+    ds = DeviceScene(sc)
+    W = sc.contributions
+    raw = ds.new_raw()
+    for _ in range(args.warmup):
+        ds.zero_c32(raw)
+        ds.run(raw=raw, chunk_snaps=args.chunk)
+    torch.cuda.synchronize()
+
+    stage_events = []
+
+    def timer(stage, a, b):
+        stage_events.append((stage, a, b))
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ds.zero_c32(raw)
+        ds.run(raw=raw, chunk_snaps=args.chunk, timer=timer)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+
+    per_stage = {}
+    echo_ms = []
+    for stage, a, b in stage_events:
+        ms = a.elapsed_time(b)
+        per_stage[stage] = per_stage.get(stage, 0.) + ms / args.steps
+        if stage == 'echo':
+            echo_ms.append(ms)
+    npol = len(sc.pols)
+    fpc = flop_per_contribution(sc.num_ch, npol)
+    n_launch = len(echo_ms)
+    contrib_per_launch = W / (n_launch / args.steps)
+    achieved = contrib_per_launch * fpc / (np.mean(echo_ms) * 1e-3) / 1e12
+
+    out = {
+        'metric': 'facet-pulse contributions/s (raw-data generation, rel. RMS <= 1e-4 vs FP64)',
+        'value': W / dt,
+        'unit': 'contributions/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': dt * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32 (FP64 range/phase/bins)',
+        'data': 'synthetic (seeded Elfouhaily surface, seeded scatterers)',
+        'config': {'workload': 'workload %d: %dx%d facets, %d pulses, %d channels, %s, K=%d'
+                               % (args.workload, sc.Nx, sc.Ny, sc.az_steps, sc.num_ch, sc.pol,
+                                  sc.K),
+                   'facets': sc.Nx * sc.Ny, 'pulses': sc.az_steps, 'channels': sc.num_ch,
+                   'range_bins': sc.rg_samp, 'contributions': W,
+                   'stage_ms_per_step': {k: round(v, 3) for k, v in per_stage.items()},
+                   'echo_launch_shape': repr(ds.shape)},
+        'roofline': {'bound': 'mfma', 'kernel': 'echo_accumulate_kernel',
+                     'achieved': achieved, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': achieved / FP32_PEAK_TFLOPS, 'traffic': None,
+                     'flop_per_contribution': fpc,
+                     'avg_launch_ms': float(np.mean(echo_ms)),
+                     'note': 'FP32 vector-ALU bound (VALU + transcendental + LDS atomics); '
+                             'peak is the MI355X FP32 figure'},
+    }
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        out['cpu_baseline'] = cpu_baseline(args.workload, args.cpu_sample, threads)
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

@@ -99,18 +99,18 @@ typedef struct osar_echo_desc {
   const double* cap_rate;   /* 2 k0 t_step c_b; Bragg phase is cap_rate * (p + 1) */
   /* per-row FP64 table [ny] */
   const double* y_row;
-  /* surface snapshots, float32 [n_snap_slots][ny][nx] */
+  /* surface snapshots, float32 [n_snap_slots][nx][ny] (x-major) */
   const float* dx;
   const float* dy;
   const float* dz;
-  /* Bragg amplitudes sqrt(RCS), float32 [n_rcs_slots][n_pol][2][ny][nx];
+  /* Bragg amplitudes sqrt(RCS), float32 [n_rcs_slots][n_pol][2][nx][ny];
      component 0 pairs with scat_m, component 1 with scat_p */
   const float* amp;
   const int32_t* rcs_slot;  /* [n_pulses_total] RCS refresh index in force */
   /* scatterer series, extended: s_ext[i] = s[i mod N], i < N + n_pulses_total */
   const osar_c32* scat_m;
   const osar_c32* scat_p;
-  const int32_t* ind0_m;    /* [ny][nx], 0 <= ind0 < N */
+  const int32_t* ind0_m;    /* [nx][ny], 0 <= ind0 < N */
   const int32_t* ind0_p;
   const float* sinc_tab;    /* [161] range response table */
   /* output: accumulated, complex64 [n_pol][n_ch][n_pulses_total][nbins] */
@@ -146,9 +146,9 @@ typedef struct osar_bragg_desc {
   const double* y_row;      /* [ny] */
   const double* spec_k;     /* [n_spec] ascending */
   const double* spec_S;     /* [n_spec] */
-  const float* diffx;       /* [ny][nx] */
+  const float* diffx;       /* [nx][ny] (x-major) */
   const float* diffy;
-  float* amp;               /* out: [n_pol][2][ny][nx] sqrt(RCS) */
+  float* amp;               /* out: [n_pol][2][nx][ny] sqrt(RCS) */
   double* rcs_sum;          /* out: [n_pol] sum of RCS over facets and components
                                (accumulated; zero it first) */
 } osar_bragg_desc;
@@ -168,7 +168,7 @@ int osar_fill_zero_f64(osar_ctx* ctx, double* p, int64_t n, void* stream);
 #define OSAR_F_DIFFXY 128
 
 typedef struct osar_surface_desc {
-  int32_t nx, ny;           /* powers of two, 16 .. 4096 */
+  int32_t nx, ny;           /* powers of two, 16 .. 2048 */
   int32_t fields;           /* OSAR_F_* mask of outputs to write */
   int32_t choppy;           /* 1: choppy slope / curvature corrections */
   int32_t n_times;          /* snapshots computed by this call */
@@ -180,7 +180,8 @@ typedef struct osar_surface_desc {
   const float* ky;          /* [ny] */
   const osar_c32* coefs;    /* [ny][nx] complex64 wave coefficients */
   const double* times;      /* [n_times] snapshot times */
-  /* outputs: float32 [n_times][ny][nx] for every requested field, NULL otherwise */
+  /* outputs: float32 [n_times][nx][ny] (x-major: field[x][y]) for every
+     requested field, NULL otherwise */
   float* out[8];            /* indexed by bit position: dx, dy, dz, diffx, diffy,
                                diffxx, diffyy, diffxy */
 } osar_surface_desc;

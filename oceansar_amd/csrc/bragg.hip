@@ -87,8 +87,9 @@ __global__ __launch_bounds__(256) void facet_bragg_kernel(osar_bragg_desc d) {
   const int npol = d.pol == 2 ? 2 : 1;
   double sums[2] = {0., 0.};
   if (idx < n) {
-    const int row = (int)(idx / d.nx);
-    const int col = (int)(idx - (size_t)row * d.nx);
+    // fields are x-major: idx = col * ny + row
+    const int col = (int)(idx / d.ny);
+    const int row = (int)(idx - (size_t)col * d.ny);
     const double t_now = d.pulse * d.t_step;
     const double az_now = (t_now - d.t_span / 2.) * d.v_ground;
     const double az = asin((d.y_row[row] - az_now) / d.gr0);

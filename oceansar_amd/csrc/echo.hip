@@ -8,10 +8,13 @@
 //
 // Work decomposition: a workgroup owns P consecutive pulses and R rows of
 // facets. Each wavefront lane is one (facet, pulse) pair: 64/P consecutive
-// columns of one row times P pulses, so the fields of a facet are loaded once
-// for P pulses and every lane of an instruction deposits into a different
-// (pulse) line or, within a line, into a tap rotated by the facet's lane
-// index, which keeps LDS atomics of neighbouring facets off the same address.
+// rows of one column times P pulses (fields are stored x-major, so those
+// facets are contiguous), so the fields of a facet are loaded once for P
+// pulses, per-column quantities are wave-uniform, and every lane of an
+// instruction deposits into a different (pulse) line or, within a line, into
+// a tap rotated by the facet's lane index: facets of one column sit in nearly
+// the same range bin, and the rotation keeps their LDS atomics on different
+// addresses.
 // Range lines of the P pulses (all channels and polarisations) live in an LDS
 // window of W bins; they are added to the raw cube with one atomic pass per
 // workgroup. Taps that fall outside the window (but inside the line) go
@@ -122,6 +125,8 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
   }
   __syncthreads();
 
+  // Lane = (facet, pulse): FPW consecutive rows of one column x times P pulses.
+  // Fields are x-major (field[x][y]), so the FPW facets of a wave are contiguous.
   const int lane = tid & 63;
   const int pi = lane & (P - 1);
   const int fsub = lane / P;
@@ -143,13 +148,20 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
   const bool bragg = (d.flags & OSAR_ECHO_BRAGG) != 0;
   const bool xti = (d.flags & OSAR_ECHO_XTI) != 0;
 
-  for (int r = 0; r < rows; ++r) {
-    const int yrow = y0 + r;
-    const double sin_az = geo[2 * (pi * R + r)];
-    const double mig = geo[2 * (pi * R + r) + 1];
-    for (int x = wave * FPW + fsub; x < d.nx; x += FPB) {
-      if (!active) continue;
-      const size_t idx = (size_t)yrow * d.nx + x;
+  for (int rg = 0; rg < rows; rg += FPB) {
+    const int r = rg + wave * FPW + fsub;
+    const bool row_ok = active && r < rows;
+    const int rr = row_ok ? r : 0;
+    const int yrow = y0 + rr;
+    const double sin_az = geo[2 * (pi * R + rr)];
+    const double mig = geo[2 * (pi * R + rr) + 1];
+    float2 fch[OSAR_MAX_CH];
+#pragma unroll
+    for (int ch = 0; ch < OSAR_MAX_CH; ++ch)
+      fch[ch] = ch < nch ? F[(pi * nch + ch) * R + rr] : make_float2(0.f, 0.f);
+    for (int x = 0; x < d.nx; ++x) {
+      if (!row_ok) continue;
+      const size_t idx = (size_t)x * d.ny + yrow;
       const size_t so = snap_off + idx;
       const double Dx = lerp_field(d.dx, so, plane, rk, w);
       const double Dy = lerp_field(d.dy, so, plane, rk, w);
@@ -181,8 +193,10 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
       }
       const bool inwin = (dbin - 4 >= wb) && (dbin + 4 <= wend);
       const int base = dbin - 4 - wb;
-      for (int ch = 0; ch < nch; ++ch) {
-        float2 f = F[(pi * nch + ch) * R + r];
+#pragma unroll
+      for (int ch = 0; ch < OSAR_MAX_CH; ++ch) {
+        if (ch >= nch) break;
+        float2 f = fch[ch];
         if (xti && d.b_xti[ch] != 0.0) {
           const double tot = d.dinc_col[x] + g.dinc;
           f = osar::cmul(f, osar::cis_fp64(-(d.k0 * d.b_xti[ch]) * tot));

@@ -1,15 +1,396 @@
-// Surface synthesis (placeholder until the FFT kernels land).
+// Surface synthesis: wave coefficients x dispersion phase -> inverse 2-D FFTs
+// -> float32 displacement / slope / curvature fields.
+//
+// Replaces the OceanSurface.t setter (oceansar/surfaces/ocean.py:418-526) for
+// the fields the raw-data path consumes ('D', 'Diff', 'Diff2'; choppy slopes).
+// Everything is FP64 (the dispersion phase omega(k) t included); outputs are
+// rounded to float32, the type OceanSurface stores them in (ocean.py:384-403).
+//
+// Two real outputs share one complex inverse FFT: for real fields
+// A = Re ifft2(mA a) and B = Re ifft2(mB a) the packed spectrum is
+// S[k] = H(mA a)[k] + i H(mB a)[k], H(g)[k] = (g[k] + conj(g[-k]))/2, and
+// ifft2(S) = A + iB. A choppy slope uses the pair (numerator, denominator) and
+// divides in the epilogue.
+//
+// Pass 1 (rows): one workgroup per row pair (j, -j): phased coefficients of
+// both rows in LDS, packed spectra built there, radix-4/2 Stockham FFT along x
+// in LDS, rows written to an FP64 workspace.
+// Pass 2 (columns): one workgroup per (packed spectrum, block of CB columns):
+// CB columns staged in LDS, FFT along y, scaled by 1/(Nx Ny) and written as
+// float32 fields in x-major layout (field[x][y]) so that every store is a
+// contiguous run along y.
+#include <math.h>
+
+#include <vector>
+
 #include "common.h"
 
+namespace {
+
+constexpr double kG = 9.80665;
+constexpr int kMaxQ = 8;
+constexpr int kMaxN = 2048;
+
+enum Mult : int {
+  M_NONE = 0,
+  M_ONE,
+  M_IKXN,    // i kx/k       (Dx)
+  M_IKYN,    // i ky/k       (Dy)
+  M_IKX,     // i kx         (Diffx)
+  M_IKY,     // i ky         (Diffy)
+  M_NKX2,    // -kx^2        (Diffxx)
+  M_NKY2,    // -ky^2        (Diffyy)
+  M_NKXKY,   // -kx ky       (Diffxy)
+  M_NKX2KI,  // -kx^2 / k    (choppy denominator x)
+  M_NKY2KI,  // -ky^2 / k    (choppy denominator y)
+};
+
+struct QPlan {
+  int ma, mb;      // multipliers of the real and imaginary component
+  int outa, outb;  // output field index (0..7) or -1
+  int choppy;      // 1: single output = Re / (1 + Im) into outa
+};
+
+struct SynthArgs {
+  int nx, ny, nq, logx, logy, cb;
+  double depth, cur_x, cur_y, thr;
+  const float* kx;
+  const float* ky;
+  const osar_c32* coefs;
+  const double* times;
+  const double2* twx;
+  const double2* twy;
+  double2* work;
+  float* out[8];
+  QPlan q[kMaxQ];
+};
+
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ double2 mult(int m, double kx, double ky, double thr) {
+  const double k = sqrt(kx * kx + ky * ky);
+  const bool good = k > thr;
+  switch (m) {
+    case M_ONE: return make_double2(1., 0.);
+    case M_IKXN: return make_double2(0., good ? kx / k : 0.);
+    case M_IKYN: return make_double2(0., good ? ky / k : 0.);
+    case M_IKX: return make_double2(0., kx);
+    case M_IKY: return make_double2(0., ky);
+    case M_NKX2: return make_double2(-(kx * kx), 0.);
+    case M_NKY2: return make_double2(-(ky * ky), 0.);
+    case M_NKXKY: return make_double2(-kx * ky, 0.);
+    case M_NKX2KI: return make_double2(good ? -(kx * kx) * (1. / k) : 0., 0.);
+    case M_NKY2KI: return make_double2(good ? -(ky * ky) * (1. / k) : 0., 0.);
+    default: return make_double2(0., 0.);
+  }
+}
+
+// H(m a)[k] = (m(k) a(k) + conj(m(-k) a(-k))) / 2
+__device__ __forceinline__ double2 herm(int m, double kx, double ky, double kxm, double kym,
+                                        double2 a, double2 am, double thr) {
+  if (m == M_NONE) return make_double2(0., 0.);
+  const double2 g = cmul2(mult(m, kx, ky, thr), a);
+  const double2 gm = cmul2(mult(m, kxm, kym, thr), am);
+  return make_double2(0.5 * (g.x + gm.x), 0.5 * (g.y - gm.y));
+}
+
+// In-LDS inverse FFT of nseq power-of-two sequences (Stockham, radix 4 with a
+// final radix-2 stage when log2 N is odd). Element e of sequence s lives at
+// buf[s * ss + e * es]. tw[k] = exp(+2 pi i k / N). Results stay in buf.
+template <int NT, int BPT>
+__device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int ss, int es,
+                         const double2* __restrict__ tw) {
+  const int tid = threadIdx.x;
+  int Ns = 1;
+  int left = logN;
+  while (left > 0) {
+    const int R = left >= 2 ? 4 : 2;
+    const int nb = N / R;
+    const int total = nseq * nb;
+    double2 v[BPT][4];
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int id = tid + b * NT;
+      if (id < total) {
+        const int s = id / nb, j = id - s * nb;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (r < R) v[b][r] = buf[s * ss + (j + r * nb) * es];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+      const int id = tid + b * NT;
+      if (id < total) {
+        const int s = id / nb, j = id - s * nb;
+        const int jm = j & (Ns - 1);
+        const int step = jm * (N / (Ns * R));
+        if (R == 4) {
+          const double2 x0 = v[b][0];
+          const double2 x1 = cmul2(v[b][1], tw[step]);
+          const double2 x2 = cmul2(v[b][2], tw[2 * step]);
+          const double2 x3 = cmul2(v[b][3], tw[3 * step]);
+          const double2 s02 = make_double2(x0.x + x2.x, x0.y + x2.y);
+          const double2 d02 = make_double2(x0.x - x2.x, x0.y - x2.y);
+          const double2 s13 = make_double2(x1.x + x3.x, x1.y + x3.y);
+          const double2 d13 = make_double2(x1.x - x3.x, x1.y - x3.y);
+          // inverse: X1 = d02 + i d13, X3 = d02 - i d13
+          const int o = (j - jm) * 4 + jm;
+          buf[s * ss + o * es] = make_double2(s02.x + s13.x, s02.y + s13.y);
+          buf[s * ss + (o + Ns) * es] = make_double2(d02.x - d13.y, d02.y + d13.x);
+          buf[s * ss + (o + 2 * Ns) * es] = make_double2(s02.x - s13.x, s02.y - s13.y);
+          buf[s * ss + (o + 3 * Ns) * es] = make_double2(d02.x + d13.y, d02.y - d13.x);
+        } else {
+          const double2 x0 = v[b][0];
+          const double2 x1 = cmul2(v[b][1], tw[step]);
+          const int o = (j - jm) * 2 + jm;
+          buf[s * ss + o * es] = make_double2(x0.x + x1.x, x0.y + x1.y);
+          buf[s * ss + (o + Ns) * es] = make_double2(x0.x - x1.x, x0.y - x1.y);
+        }
+      }
+    }
+    __syncthreads();
+    Ns *= R;
+    left -= (R == 4 ? 2 : 1);
+  }
+}
+
+constexpr int kRowThreads = 256;
+constexpr int kColThreads = 512;
+
+__global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double2* A = (double2*)lds_raw;  // [2][nx] phased coefficients of rows j, jm
+  double2* S = A + 2 * a.nx;       // [2][nx] packed spectrum
+  const int j = blockIdx.x;
+  const int jm = (a.ny - j) % a.ny;
+  const int nrow = (j == jm) ? 1 : 2;
+  const int ti = blockIdx.y;
+  const double t = a.times[ti];
+  const int tid = threadIdx.x;
+  const int nx = a.nx;
+
+  for (int e = tid; e < nrow * nx; e += kRowThreads) {
+    const int r = e / nx, i = e - r * nx;
+    const int row = r == 0 ? j : jm;
+    const double kx = (double)a.kx[i], ky = (double)a.ky[row];
+    const double k = sqrt(kx * kx + ky * ky);
+    const double om = a.depth > 0. ? sqrt(kG * k * tanh(k * a.depth)) : sqrt(kG * k);
+    double sn, cs;
+    sincos(-(om * t), &sn, &cs);
+    const osar_c32 c = a.coefs[(size_t)row * nx + i];
+    A[r * nx + i] = cmul2(make_double2((double)c.re, (double)c.im), make_double2(cs, sn));
+  }
+  __syncthreads();
+  for (int qi = 0; qi < a.nq; ++qi) {
+    const QPlan q = a.q[qi];
+    for (int e = tid; e < nrow * nx; e += kRowThreads) {
+      const int r = e / nx, i = e - r * nx;
+      const int rm = (nrow == 1) ? 0 : 1 - r;
+      const int row = r == 0 ? j : jm, rowm = r == 0 ? jm : j;
+      const int im = (nx - i) & (nx - 1);
+      const double kx = (double)a.kx[i], ky = (double)a.ky[row];
+      const double kxm = (double)a.kx[im], kym = (double)a.ky[rowm];
+      const double2 ak = A[r * nx + i], am = A[rm * nx + im];
+      const double2 ha = herm(q.ma, kx, ky, kxm, kym, ak, am, a.thr);
+      const double2 hb = herm(q.mb, kx, ky, kxm, kym, ak, am, a.thr);
+      S[r * nx + i] = make_double2(ha.x - hb.y, ha.y + hb.x);  // ha + i hb
+    }
+    __syncthreads();
+    lds_ifft<kRowThreads, 8>(S, nx, a.logx, nrow, nx, 1, a.twx);
+    double2* w = a.work + ((size_t)(ti * a.nq + qi) * a.ny) * nx;
+    for (int e = tid; e < nrow * nx; e += kRowThreads) {
+      const int r = e / nx, i = e - r * nx;
+      const int row = r == 0 ? j : jm;
+      w[(size_t)row * nx + i] = S[r * nx + i];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kColThreads) void synth_cols_kernel(SynthArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  double2* C = (double2*)lds_raw;  // [ny][cb]
+  const int cb = a.cb;
+  const int x0 = blockIdx.x * cb;
+  const int qi = blockIdx.y;
+  const int ti = blockIdx.z;
+  const int tid = threadIdx.x;
+  const int ny = a.ny, nx = a.nx;
+  const double2* w = a.work + ((size_t)(ti * a.nq + qi) * ny) * nx;
+  for (int e = tid; e < ny * cb; e += kColThreads) {
+    const int y = e / cb, c = e - y * cb;
+    C[e] = w[(size_t)y * nx + x0 + c];
+  }
+  __syncthreads();
+  lds_ifft<kColThreads, 8>(C, ny, a.logy, cb, 1, cb, a.twy);
+  const QPlan q = a.q[qi];
+  const double scale = 1.0 / ((double)nx * (double)ny);
+  const double t = a.times[ti];
+  const size_t plane = (size_t)nx * ny;
+  for (int e = tid; e < ny * cb; e += kColThreads) {
+    const int c = e / ny, y = e - c * ny;  // consecutive threads walk y: contiguous stores
+    const double2 z = C[y * cb + c];
+    const double re = z.x * scale, im = z.y * scale;
+    const size_t o = (size_t)ti * plane + (size_t)(x0 + c) * ny + y;
+    if (q.choppy) {
+      a.out[q.outa][o] = (float)(re / (1. + im));
+      continue;
+    }
+    if (q.outa >= 0) {
+      double v = re;
+      if (q.outa == 0) v = v + a.cur_x * t;
+      if (q.outa == 1) v = v + a.cur_y * t;
+      a.out[q.outa][o] = (float)v;
+    }
+    if (q.outb >= 0) {
+      double v = im;
+      if (q.outb == 0) v = v + a.cur_x * t;
+      if (q.outb == 1) v = v + a.cur_y * t;
+      a.out[q.outb][o] = (float)v;
+    }
+  }
+}
+
+int ilog2(int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return (1 << l) == n ? l : -1;
+}
+
+int make_plan(const osar_surface_desc* d, QPlan* q, int* nq) {
+  // Real outputs in pairing order with their multipliers.
+  struct Item { int out, m; };
+  static const int kMult[8] = {M_IKXN, M_IKYN, M_ONE, M_IKX, M_IKY, M_NKX2, M_NKY2, M_NKXKY};
+  static const int kOrder[8] = {2, 0, 1, 3, 4, 5, 6, 7};  // Dz, Dx, Dy, Diffx, Diffy, ...
+  std::vector<Item> items;
+  int n = 0;
+  if (d->choppy && (d->fields & (OSAR_F_DIFFXX | OSAR_F_DIFFYY | OSAR_F_DIFFXY))) {
+    osar_set_error("choppy second derivatives are not supported by osar_surface_synth");
+    return OSAR_EUNSUPPORTED;
+  }
+  for (int oi = 0; oi < 8; ++oi) {
+    const int f = kOrder[oi];
+    if (!(d->fields & (1 << f))) continue;
+    if (d->choppy && (f == 3 || f == 4)) {
+      q[n++] = QPlan{f == 3 ? M_IKX : M_IKY, f == 3 ? M_NKX2KI : M_NKY2KI, f, -1, 1};
+      continue;
+    }
+    items.push_back(Item{f, kMult[f]});
+  }
+  for (size_t i = 0; i < items.size(); i += 2) {
+    const bool two = i + 1 < items.size();
+    q[n++] = QPlan{items[i].m, two ? items[i + 1].m : (int)M_NONE, items[i].out,
+                   two ? items[i + 1].out : -1, 0};
+  }
+  *nq = n;
+  return OSAR_OK;
+}
+
+struct TwCache {
+  int n = 0;
+  double2* dev = nullptr;
+};
+
+// Twiddle tables exp(+2 pi i k / N), computed on the host in long double.
+int twiddles(osar_ctx* ctx, int n, const double2** out) {
+  static thread_local TwCache cache[64][16];
+  const int l = ilog2(n);
+  OSAR_REQUIRE(ctx->device >= 0 && ctx->device < 64 && l >= 0 && l < 16, OSAR_EUNSUPPORTED,
+               "twiddle table for device %d, n %d", ctx->device, n);
+  TwCache& c = cache[ctx->device][l];
+  if (c.n != n || c.dev == nullptr) {
+    std::vector<double2> h(n);
+    const long double pi = 3.14159265358979323846264338327950288L;
+    for (int k = 0; k < n; ++k) {
+      const long double ang = 2.0L * pi * (long double)k / (long double)n;
+      h[k] = make_double2((double)cosl(ang), (double)sinl(ang));
+    }
+    double2* p = nullptr;
+    OSAR_HIP(hipMalloc(&p, sizeof(double2) * n));
+    OSAR_HIP(hipMemcpy(p, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    c.n = n;
+    c.dev = p;
+  }
+  *out = c.dev;
+  return OSAR_OK;
+}
+
+}  // namespace
+
 extern "C" int64_t osar_surface_workspace_bytes(const osar_surface_desc* d) {
-  (void)d;
-  return 0;
+  if (!d) return -1;
+  QPlan q[kMaxQ];
+  int nq = 0;
+  if (make_plan(d, q, &nq) != OSAR_OK) return -1;
+  return (int64_t)nq * d->n_times * (int64_t)d->nx * d->ny * (int64_t)sizeof(double2);
 }
 
 extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream) {
-  (void)ctx;
-  (void)d;
-  (void)stream;
-  osar_set_error("osar_surface_synth: not implemented in this build");
-  return OSAR_EUNSUPPORTED;
+  OSAR_PTR(ctx);
+  OSAR_PTR(d);
+  const int lx = ilog2(d->nx), ly = ilog2(d->ny);
+  OSAR_REQUIRE(lx >= 4 && ly >= 4 && d->nx <= kMaxN && d->ny <= kMaxN, OSAR_EUNSUPPORTED,
+               "surface %dx%d: sizes must be powers of two in [16, %d]", d->nx, d->ny, kMaxN);
+  OSAR_REQUIRE(d->n_times >= 1, OSAR_EINVAL, "n_times must be >= 1");
+  OSAR_PTR(d->kx);
+  OSAR_PTR(d->ky);
+  OSAR_PTR(d->coefs);
+  OSAR_PTR(d->times);
+  SynthArgs a{};
+  OSAR_REQUIRE(make_plan(d, a.q, &a.nq) == OSAR_OK, OSAR_EUNSUPPORTED, "%s", osar_last_error());
+  if (a.nq == 0) return OSAR_OK;
+  for (int f = 0; f < 8; ++f) {
+    if (d->fields & (1 << f)) {
+      OSAR_REQUIRE(d->out[f] != nullptr, OSAR_EINVAL, "output field %d requested but NULL", f);
+    }
+    a.out[f] = d->out[f];
+  }
+  OSAR_HIP(hipSetDevice(ctx->device));
+  const size_t need = (size_t)osar_surface_workspace_bytes(d);
+  if (need > ctx->work_bytes) {
+    if (ctx->work) OSAR_HIP(hipFree(ctx->work));
+    ctx->work = nullptr;
+    ctx->work_bytes = 0;
+    OSAR_HIP(hipMalloc(&ctx->work, need));
+    ctx->work_bytes = need;
+  }
+  a.nx = d->nx;
+  a.ny = d->ny;
+  a.logx = lx;
+  a.logy = ly;
+  a.depth = d->depth;
+  a.cur_x = d->current_x;
+  a.cur_y = d->current_y;
+  a.thr = (d->kx_res < d->ky_res ? d->kx_res : d->ky_res) / 2.0;
+  a.kx = d->kx;
+  a.ky = d->ky;
+  a.coefs = d->coefs;
+  a.times = d->times;
+  a.work = (double2*)ctx->work;
+  int rc = twiddles(ctx, d->nx, &a.twx);
+  if (rc) return rc;
+  rc = twiddles(ctx, d->ny, &a.twy);
+  if (rc) return rc;
+  int cb = (8192) / d->ny;
+  if (cb > 8) cb = 8;
+  if (cb > d->nx) cb = d->nx;
+  a.cb = cb;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t lds_rows = 4 * (size_t)d->nx * sizeof(double2);
+  const size_t lds_cols = (size_t)cb * d->ny * sizeof(double2);
+  OSAR_HIP(hipFuncSetAttribute((const void*)synth_rows_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows));
+  OSAR_HIP(hipFuncSetAttribute((const void*)synth_cols_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cols));
+  hipLaunchKernelGGL(synth_rows_kernel, dim3(d->ny / 2 + 1, d->n_times), dim3(kRowThreads),
+                     lds_rows, s, a);
+  OSAR_LAUNCHED("synth_rows_kernel");
+  hipLaunchKernelGGL(synth_cols_kernel, dim3(d->nx / cb, a.nq, d->n_times), dim3(kColThreads),
+                     lds_cols, s, a);
+  OSAR_LAUNCHED("synth_cols_kernel");
+  return OSAR_OK;
 }

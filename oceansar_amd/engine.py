@@ -55,8 +55,9 @@ class DeviceScene:
             self.cap_rate = _t(sc.cap_rate, f64, dev)
             self.scat_m = _t(self._extend(sc.scat_m), c64, dev)
             self.scat_p = _t(self._extend(sc.scat_p), c64, dev)
-            self.ind0_m = _t(np.mod(sc.scat_m.ind0, sc.scat_m.N), i32, dev)
-            self.ind0_p = _t(np.mod(sc.scat_p.ind0, sc.scat_p.N), i32, dev)
+            # x-major [nx][ny], like every per-facet device array
+            self.ind0_m = _t(np.mod(sc.scat_m.ind0, sc.scat_m.N).T, i32, dev)
+            self.ind0_p = _t(np.mod(sc.scat_p.ind0, sc.scat_p.N).T, i32, dev)
             self.rcs_slot = _t(sc.rcs_slot, i32, dev)
             self.spec_k = _t(sc.bragg.spec_k, f64, dev)
             self.spec_S = _t(sc.bragg.spec_S, f64, dev)
@@ -187,3 +188,137 @@ class DeviceScene:
         d = self.echo_desc(dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape)
         _lib.check(self.ctx.lib.osar_echo_accumulate(self.ctx.h, ctypes.byref(d),
                                                      _lib.stream_ptr()))
+
+    # ------------------------------------------------------------------
+    def fields_needed(self):
+        f = ['Dx', 'Dy', 'Dz']
+        if self.sc.bragg_enable:
+            f += ['Diffx', 'Diffy']
+        if self.sc.spec_enable:
+            f += ['Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
+        return [n for n in _lib.FIELD_ORDER if n in f]
+
+    def synth(self, times, out, fields=None):
+        """Surface fields at `times` into out[name] = float32 [n_times][nx][ny] (x-major)."""
+        import torch
+        sc = self.sc
+        surf = sc.surface
+        fields = fields or list(out.keys())
+        if getattr(surf, 'swell_enable', False):
+            raise NotImplementedError('monochromatic swell is not supported on the GPU path')
+        times_t = torch.as_tensor(np.asarray(times, dtype=np.float64), device=self.dev)
+        d = _lib.SurfaceDesc()
+        d.nx, d.ny = sc.Nx, sc.Ny
+        mask = 0
+        for n in fields:
+            mask |= _lib.FIELD_BITS[n]
+        d.fields = mask
+        d.choppy = 1 if surf.choppy else 0
+        d.n_times = len(times)
+        d.depth = -1.0 if surf.depth is None else float(surf.depth)
+        d.current_x, d.current_y = float(surf.current[0]), float(surf.current[1])
+        d.kx_res, d.ky_res = float(surf.kx_res), float(surf.ky_res)
+        d.kx, d.ky, d.coefs = _lib.ptr(self.kx), _lib.ptr(self.ky), _lib.ptr(self.coefs)
+        d.times = _lib.ptr(times_t)
+        for i, n in enumerate(_lib.FIELD_ORDER):
+            if n in fields:
+                t = out[n]
+                assert t.dtype == torch.float32 and t.is_contiguous()
+                assert tuple(t.shape[-2:]) == (sc.Nx, sc.Ny) and t.shape[0] >= len(times)
+                d.out[i] = t.data_ptr()
+        _lib.check(self.ctx.lib.osar_surface_synth(self.ctx.h, ctypes.byref(d), _lib.stream_ptr()))
+        self._keep = times_t   # alive until the stream has consumed it
+
+    # ------------------------------------------------------------------
+    def run(self, p0=0, p1=None, chunk_snaps=32, raw=None, shape=None, timer=None):
+        """GPU raw cube for pulses [p0, p1): returns (raw device tensor, NRCS_avg numpy).
+
+        Snapshots are synthesised in chunks of `chunk_snaps` intervals (the
+        boundary snapshot of a chunk is recomputed by the next one), Bragg
+        amplitudes at every refresh pulse of the chunk, then one accumulator
+        launch covers the chunk. `timer`, if given, is called as
+        timer(stage, start_event, end_event) for every launch group.
+        """
+        import torch
+        sc = self.sc
+        p1 = self.n_pulses if p1 is None else p1
+        K = sc.K
+        raw = self.new_raw() if raw is None else raw
+        names = self.fields_needed()
+        nx, ny = sc.Nx, sc.Ny
+        C = max(1, int(chunk_snaps))
+        buf = {n: torch.empty((C + 1, nx, ny), dtype=torch.float32, device=self.dev)
+               for n in names}
+        nref = len(sc.rcs_refresh) if sc.bragg_enable else 0
+        rcs_sum = torch.zeros(max(nref, 1), self.npol, dtype=torch.float64, device=self.dev)
+        amp = None
+        if sc.bragg_enable:
+            amp = torch.empty((C + 2, self.npol, 2, nx, ny), dtype=torch.float32, device=self.dev)
+        held = -1           # refresh index whose amplitudes sit in amp[0]
+        p = p0
+        while p < p1:
+            m0 = p // K
+            m1 = min(m0 + C, (p1 - 1) // K + 1)       # last snapshot needed
+            q1 = min(m1 * K, p1) if m1 > m0 else p1
+            if q1 <= p:
+                q1 = min(p + K, p1)
+            last = q1 - 1
+            m_last = last // K + (1 if last % K else 0)
+            times = [sc.snapshot_time(m) for m in range(m0, m_last + 1)]
+            ev = self._events(timer)
+            self.synth(times, buf, names)
+            self._mark(timer, 'synth', ev)
+            r0 = r1 = 0
+            if sc.bragg_enable:
+                r0 = int(sc.rcs_slot[p])
+                r1 = int(sc.rcs_slot[last])
+                ev = self._events(timer)
+                for r in range(r0, r1 + 1):
+                    if r == held:
+                        continue            # computed by the previous chunk, kept in amp[0]
+                    pr = int(sc.rcs_refresh[r])
+                    mi = pr // K - m0
+                    if mi < 0:
+                        # refresh pulse before this chunk: synthesise its slopes
+                        tmp = {n: torch.empty((1, nx, ny), dtype=torch.float32, device=self.dev)
+                               for n in ('Diffx', 'Diffy')}
+                        self.synth([sc.snapshot_time(pr // K)], tmp, ['Diffx', 'Diffy'])
+                        dxs, dys = tmp['Diffx'][0], tmp['Diffy'][0]
+                    else:
+                        dxs, dys = buf['Diffx'][mi], buf['Diffy'][mi]
+                    self.bragg(dxs, dys, pr, amp[r - r0], rcs_sum[r])
+                self._mark(timer, 'bragg', ev)
+            ev = self._events(timer)
+            self.accumulate(buf['Dx'], buf['Dy'], buf['Dz'], m0, amp, r0, p, q1 - p, raw,
+                            shape)
+            self._mark(timer, 'echo', ev)
+            if sc.bragg_enable:
+                # the last refresh in force stays in amp[0] for the next chunk
+                if r1 > r0:
+                    amp[0].copy_(amp[r1 - r0])
+                held = r1
+            p = q1
+        nrcs = np.zeros((self.npol, self.n_pulses))
+        if sc.bragg_enable:
+            sums = rcs_sum.cpu().numpy()
+            slots = sc.rcs_slot[p0:p1]
+            nrcs[:, p0:p1] = sums[slots].T / sc.surface_area
+        return raw, nrcs
+
+    @staticmethod
+    def _events(timer):
+        if timer is None:
+            return None
+        import torch
+        a = torch.cuda.Event(enable_timing=True)
+        a.record()
+        return a
+
+    @staticmethod
+    def _mark(timer, stage, start):
+        if timer is None:
+            return
+        import torch
+        b = torch.cuda.Event(enable_timing=True)
+        b.record()
+        timer(stage, start, b)

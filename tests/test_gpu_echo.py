@@ -36,7 +36,9 @@ def _snapshots(sc, m0, m1):
     snaps = [osurf.synth(sc.surface, sc.snapshot_time(m), ('D', 'Diff')) for m in range(m0, m1 + 1)]
     out = {}
     for n in ('Dx', 'Dy', 'Dz', 'Diffx', 'Diffy'):
-        out[n] = torch.as_tensor(np.stack([s[n] for s in snaps]), device='cuda')
+        # device fields are x-major: [snapshot][x][y]
+        out[n] = torch.as_tensor(np.ascontiguousarray(np.stack([s[n].T for s in snaps])),
+                                 device='cuda')
     return snaps, out
 
 
@@ -48,14 +50,14 @@ def test_bragg_kernel_matches_oracle(need_gpu, cfg1_scene):
     ds = DeviceScene(sc)
     snaps, dev = _snapshots(sc, 25, 25)
     p = 25 * sc.K
-    amp = torch.empty((1, 2, sc.Ny, sc.Nx), dtype=torch.float32, device='cuda')
+    amp = torch.empty((1, 2, sc.Nx, sc.Ny), dtype=torch.float32, device='cuda')
     tot = torch.zeros(1, dtype=torch.float64, device='cuda')
     ds.bragg(dev['Diffx'][0], dev['Diffy'][0], p, amp, tot)
     cache = osar.SnapshotCache(sc)
     cache.cache[25] = snaps[0]
     rcs = osar.bragg_rcs(sc, cache, p)['vv']
     ref = np.sqrt(rcs)
-    got = amp[0].cpu().numpy()
+    got = amp[0].cpu().numpy().transpose(0, 2, 1)
     assert np.all(np.isfinite(got))
     err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)
     assert np.max(err[ref > 1e-12]) < 2e-6
@@ -72,7 +74,7 @@ def test_accumulator_matches_oracle(need_gpu, cfg1_scene):
     snaps, dev = _snapshots(sc, m0, m1)
     ds = DeviceScene(sc)
     r0, r1 = int(sc.rcs_slot[p0]), int(sc.rcs_slot[p0 + n - 1])
-    amp = torch.empty((r1 - r0 + 1, 1, 2, sc.Ny, sc.Nx), dtype=torch.float32, device='cuda')
+    amp = torch.empty((r1 - r0 + 1, 1, 2, sc.Nx, sc.Ny), dtype=torch.float32, device='cuda')
     for r in range(r0, r1 + 1):
         pr = int(sc.rcs_refresh[r])
         ds.bragg(dev['Diffx'][pr // sc.K - m0], dev['Diffy'][pr // sc.K - m0], pr, amp[r - r0])
@@ -103,7 +105,7 @@ def test_accumulator_launch_shapes_agree(need_gpu, cfg1_scene):
     _, dev = _snapshots(sc, m0, m1)
     ds = DeviceScene(sc)
     r0, r1 = int(sc.rcs_slot[p0]), int(sc.rcs_slot[p0 + n - 1])
-    amp = torch.empty((r1 - r0 + 1, 1, 2, sc.Ny, sc.Nx), dtype=torch.float32, device='cuda')
+    amp = torch.empty((r1 - r0 + 1, 1, 2, sc.Nx, sc.Ny), dtype=torch.float32, device='cuda')
     for r in range(r0, r1 + 1):
         pr = int(sc.rcs_refresh[r])
         ds.bragg(dev['Diffx'][pr // sc.K - m0], dev['Diffy'][pr // sc.K - m0], pr, amp[r - r0])
