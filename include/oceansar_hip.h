@@ -65,6 +65,9 @@ int64_t osar_struct_size(int which);
 /* Flags of osar_echo_desc.flags */
 #define OSAR_ECHO_BRAGG 1      /* add the two Bragg components */
 #define OSAR_ECHO_XTI 2        /* apply the cross-track phase (any b_xti != 0) */
+#define OSAR_ECHO_SPECULAR 4   /* add the Kirchhoff stationary-phase specular field
+                                  (RCSKA.field 'spa', oceansar/nrcs/ka.py:105-284,
+                                  used at sar_raw_nompi.py:401-428) */
 
 typedef struct osar_echo_desc {
   /* sizes */
@@ -83,7 +86,7 @@ typedef struct osar_echo_desc {
   int32_t pulses_per_block; /* 1, 2, 4 or 8 */
   int32_t rows_per_block;
   int32_t window_bins;      /* LDS range-line window per (pol, ch, pulse) */
-  int32_t reserved;
+  int32_t pol;              /* 0 = vv, 1 = hh, 2 = DP (line 0 hh, line 1 vv) */
   /* FP64 scalars */
   double t_step, t_span, v_ground, sr0, gr0, k0;
   double min_sr, binsize;   /* bin position x = (sr - min_sr) / binsize */
@@ -115,6 +118,18 @@ typedef struct osar_echo_desc {
   const float* sinc_tab;    /* [161] range response table */
   /* output: accumulated, complex64 [n_pol][n_ch][n_pulses_total][nbins] */
   osar_c32* raw;
+  /* specular (OSAR_ECHO_SPECULAR): slope and curvature snapshots, float32
+     [n_snap_slots][nx][ny] like dx/dy/dz; incidence per column [nx]; facet size */
+  const float* sx;          /* Diffx */
+  const float* sy;          /* Diffy */
+  const float* sxx;         /* Diffxx */
+  const float* syy;         /* Diffyy */
+  const float* sxy;         /* Diffxy */
+  const double* inc_col;
+  double facet_dx, facet_dy;
+  /* out: accumulated sum of |E_spec|^2 per pulse [n_pulses_total], of the last
+     polarisation (vv when present), as the driver's NRCS_avg does */
+  double* spec_energy;
 } osar_echo_desc;
 
 int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void* stream);

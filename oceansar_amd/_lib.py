@@ -14,6 +14,7 @@ MAX_CH = 8
 
 ECHO_BRAGG = 1
 ECHO_XTI = 2
+ECHO_SPECULAR = 4
 
 
 class OsarError(RuntimeError):
@@ -28,14 +29,16 @@ class EchoDesc(ctypes.Structure):
     _fields_ = [(n, c_int32) for n in (
         'nx', 'ny', 'n_pulses_total', 'pulse0', 'npulses', 'nbins', 'n_ch', 'n_pol', 'K',
         'snap0', 'n_snap_slots', 'rcs0', 'n_rcs_slots', 'flags', 'pulses_per_block',
-        'rows_per_block', 'window_bins', 'reserved')] + \
+        'rows_per_block', 'window_bins', 'pol')] + \
         [(n, c_double) for n in (
             't_step', 't_span', 'v_ground', 'sr0', 'gr0', 'k0', 'min_sr', 'binsize',
             'ltx_over_l', 'lrx_over_l', 'sr_col_min', 'sr_col_max', 'resid_max')] + \
         [('b_ati', c_double * MAX_CH), ('b_xti', c_double * MAX_CH)] + \
         [(n, c_void_p) for n in (
             'sr_col', 'sin_inc', 'cos_inc', 'dinc_col', 'cap_rate', 'y_row', 'dx', 'dy', 'dz',
-            'amp', 'rcs_slot', 'scat_m', 'scat_p', 'ind0_m', 'ind0_p', 'sinc_tab', 'raw')]
+            'amp', 'rcs_slot', 'scat_m', 'scat_p', 'ind0_m', 'ind0_p', 'sinc_tab', 'raw',
+            'sx', 'sy', 'sxx', 'syy', 'sxy', 'inc_col')] + \
+        [('facet_dx', c_double), ('facet_dy', c_double), ('spec_energy', c_void_p)]
 
 
 class BraggDesc(ctypes.Structure):

@@ -70,14 +70,132 @@ __device__ __forceinline__ double lerp_field(const float* f, size_t o0, size_t s
   return (1. - w) * a + w * b;
 }
 
-template <int P>
+// ---- Kirchhoff stationary-phase specular field (RCSKA.field, 'spa', monostatic;
+// oceansar/nrcs/ka.py:131-270), FP64, in the reference's operation order.
+struct zd {
+  double re, im;
+};
+__device__ __forceinline__ zd zmul(zd a, zd b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ zd zscl(zd a, double s) { return {a.re * s, a.im * s}; }
+__device__ __forceinline__ zd zadd(zd a, zd b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ zd zsub(zd a, zd b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ zd zdiv(zd a, zd b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double rat = b.im / b.re, scl = 1.0 / (b.re + b.im * rat);
+    return {(a.re + a.im * rat) * scl, (a.im - a.re * rat) * scl};
+  }
+  const double rat = b.re / b.im, scl = 1.0 / (b.im + b.re * rat);
+  return {(a.re * rat + a.im) * scl, (a.im * rat - a.re) * scl};
+}
+__device__ __forceinline__ zd zsqrt(zd z) {
+  // principal square root
+  const double m = hypot(z.re, z.im);
+  if (z.re >= 0.) {
+    const double t = sqrt(0.5 * (m + z.re));
+    return {t, z.im / (2. * t)};
+  }
+  const double t = sqrt(0.5 * (m - z.re));
+  return {fabs(z.im) / (2. * t), z.im >= 0. ? t : -t};
+}
+struct v3 {
+  double x, y, z;
+};
+__device__ __forceinline__ v3 cross3(v3 a, v3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double dot3(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+struct zv3 {
+  zd x, y, z;
+};
+__device__ __forceinline__ zv3 zv(zd s, v3 v) {
+  return {zscl(s, v.x), zscl(s, v.y), zscl(s, v.z)};
+}
+__device__ __forceinline__ zv3 zvsub(zv3 a, zv3 b) {
+  return {zsub(a.x, b.x), zsub(a.y, b.y), zsub(a.z, b.z)};
+}
+__device__ __forceinline__ zv3 zvadd(zv3 a, zv3 b) {
+  return {zadd(a.x, b.x), zadd(a.y, b.y), zadd(a.z, b.z)};
+}
+// cross(real a, complex b)
+__device__ __forceinline__ zv3 rzcross(v3 a, zv3 b) {
+  return {zsub(zscl(b.z, a.y), zscl(b.y, a.z)), zsub(zscl(b.x, a.z), zscl(b.z, a.x)),
+          zsub(zscl(b.y, a.x), zscl(b.x, a.y))};
+}
+
+// Returns true for a stationary-phase facet and writes E for the requested
+// polarisations (pol_code 0 = vv, 1 = hh, 2 = both: e[0] = hh, e[1] = vv).
+__device__ bool ka_spa(double k0, int pol_code, double th, double ph, double Dx, double Dy,
+                       double Dxx, double Dyy, double Dxy, double fdx, double fdy, zd* e) {
+#pragma clang fp contract(off)
+  const double sin_t = sin(th), cos_t = cos(th), tan_t = tan(th);
+  const double sin_p = sin(ph), cos_p = cos(ph);
+  double J = Dxx * Dyy - Dxy * Dxy;
+  if (J == 0.) return false;                           // reference: J -> NaN, never sps
+  const double J_abs = fabs(J);
+  const double delta_x = (1. / J_abs) * (Dxy * (Dy - tan_t * sin_p) - Dyy * (Dx - tan_t * cos_p));
+  const double delta_y = (1. / J_abs) * (Dxy * (Dx - tan_t * cos_p) - Dxx * (Dy - tan_t * sin_p));
+  if (!((-0.5 * fdx < delta_x) && (delta_x < 0.5 * fdx) && (-0.5 * fdy < delta_y) &&
+        (delta_y < 0.5 * fdy)))
+    return false;
+  const double delta_z = delta_x * Dx + delta_y * Dy;
+  const zd eps_sgn = J > 0. ? zd{Dxx > 0. ? 1. : (Dxx < 0. ? -1. : 0.), 0.} : zd{0., 1.};
+  const double n_norm = sqrt(Dx * Dx + Dy * Dy + 1.);
+  const v3 n = {-Dx / n_norm, -Dy / n_norm, 1. / n_norm};
+  const v3 n_i = {sin_t * cos_p, sin_t * sin_p, -cos_t};
+  const v3 n_s = {-sin_t * cos_p, -sin_t * sin_p, cos_t};
+  const v3 q = {k0 * (n_s.x - n_i.x), k0 * (n_s.y - n_i.y), k0 * (n_s.z - n_i.z)};
+  v3 t = cross3(n_i, n);
+  const double tn = sqrt((t.x * t.x + t.y * t.y) + t.z * t.z);
+  t = {t.x / tn, t.y / tn, t.z / tn};
+  const v3 dd = cross3(n_i, t);
+  const double cos_l = -dot3(n, n_i);
+  const zd eps = {73., 18.};
+  const zd se = zsqrt({73. + cos_l * cos_l - 1., 18.});
+  const zd ec = zscl(eps, cos_l);
+  const zd r_v = zdiv(zsub(ec, se), zadd(ec, se));
+  const zd r_h = zdiv(zsub({cos_l, 0.}, se), zadd({cos_l, 0.}, se));
+  const double n_p_ni = -cos_l;
+  const v3 n_x_t = cross3(n, t);
+  const v3 h_i = {-sin_p, cos_p, 0.};
+  const v3 v_i = {-cos_t * cos_p, -cos_t * sin_p, -sin_t};
+  const zd K = zdiv(zmul({0., -k0}, {cos(-k0 * 2.), sin(-k0 * 2.)}),
+                    {(4. * M_PI) * (4. * M_PI), 0.});
+  const double qd = (q.x * delta_x + q.y * delta_y) + q.z * delta_z;
+  const zd eq = {cos(qd), sin(qd)};
+  const double tail = 2. * M_PI / q.z * n_norm / sqrt(J_abs);
+  const int first = pol_code == 2 ? 0 : pol_code;   // 0 = vv, 1 = hh in single-pol mode
+  const int npol = pol_code == 2 ? 2 : 1;
+  for (int ip = 0; ip < npol; ++ip) {
+    const bool is_h = pol_code == 2 ? (ip == 0) : (first == 1);
+    const v3 a = is_h ? h_i : v_i;
+    const double a_t = dot3(a, t), a_d = dot3(a, dd);
+    // n x Es = ((1 + r_h) a_t) n_x_t - ((1 - r_v) n_p_ni a_d) t
+    const zv3 nxEs = zvsub(zv(zscl(zadd({1., 0.}, r_h), a_t), n_x_t),
+                           zv(zscl(zscl(zsub({1., 0.}, r_v), n_p_ni), a_d), t));
+    // eta n x Hs = -(((1 - r_h) n_p_ni a_t) t + ((1 + r_v) a_d) n_x_t)
+    const zv3 eHs0 = zvadd(zv(zscl(zscl(zsub({1., 0.}, r_h), n_p_ni), a_t), t),
+                           zv(zscl(zadd({1., 0.}, r_v), a_d), n_x_t));
+    const zv3 eHs = {zscl(eHs0.x, -1.), zscl(eHs0.y, -1.), zscl(eHs0.z, -1.)};
+    const zv3 inner = zvsub(nxEs, rzcross(n_s, eHs));
+    const zv3 c = rzcross(n_s, inner);
+    const zd pp = zadd(zadd(zscl(c.x, a.x), zscl(c.y, a.y)), zscl(c.z, a.z));
+    zd E = zmul(zmul(zmul(eps_sgn, pp), K), eq);
+    E = zscl(E, tail);
+    e[ip] = E;
+  }
+  return true;
+}
+
+template <int P, bool SPEC>
 __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_desc d, int W,
                                                                    int R) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int npol = d.n_pol, nch = d.n_ch;
   const int nlines = npol * nch * P;
-  double* geo = (double*)lds_raw;                        // [P][R][2] sin_az, mig
-  float2* F = (float2*)(geo + 2 * P * R);                // [P][nch][R]
+  double* geo = (double*)lds_raw;                        // [P][R][3] sin_az, mig, az_proj
+  float2* F = (float2*)(geo + 3 * P * R);                // [P][nch][R]
   float* acc = (float*)(F + P * nch * R);                // [nlines][2][W]
   float* tab = acc + (size_t)nlines * 2 * W;             // [kTab]
   int* wlo = (int*)(tab + 164);                          // [P]
@@ -93,7 +211,7 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
   for (int i = tid; i < P * R; i += kThreads) {
     const int pi = i / R, r = i - pi * R;
     const int p = pstart + pi;
-    double sin_az = 0., mig = 0.;
+    double sin_az = 0., mig = 0., az_proj = 0.;
     double beam = 0.;
     if (p < pend && r < rows) {
       const double t_now = p * d.t_step;
@@ -101,10 +219,12 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
       const double az = d.y_row[y0 + r] - az_now;
       sin_az = az / d.sr0;
       mig = (az / 2) * sin_az;
+      if (SPEC) az_proj = asin(az / d.gr0);
       beam = np_sinc(d.ltx_over_l * sin_az) * np_sinc(d.lrx_over_l * sin_az);
     }
-    geo[2 * i] = sin_az;
-    geo[2 * i + 1] = mig;
+    geo[3 * i] = sin_az;
+    geo[3 * i + 1] = mig;
+    geo[3 * i + 2] = az_proj;
     for (int ch = 0; ch < nch; ++ch) {
       const double ph = -(d.k0 * d.b_ati[ch]) * sin_az;
       double s, c;
@@ -116,7 +236,7 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
   if (tid < P) {
     double mmin = 1e300, mmax = -1e300;
     for (int r = 0; r < rows; ++r) {
-      const double m = geo[2 * (tid * R + r) + 1];
+      const double m = geo[3 * (tid * R + r) + 1];
       mmin = fmin(mmin, m);
       mmax = fmax(mmax, m);
     }
@@ -147,14 +267,18 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
   const double two_k0 = 2 * d.k0;
   const bool bragg = (d.flags & OSAR_ECHO_BRAGG) != 0;
   const bool xti = (d.flags & OSAR_ECHO_XTI) != 0;
+  // pol code of the specular field: 0 vv, 1 hh, 2 DP (e[0] hh, e[1] vv)
+  const int spol = npol == 2 ? 2 : (d.pol == 1 ? 1 : 0);
+  double spec_acc = 0.;
 
   for (int rg = 0; rg < rows; rg += FPB) {
     const int r = rg + wave * FPW + fsub;
     const bool row_ok = active && r < rows;
     const int rr = row_ok ? r : 0;
     const int yrow = y0 + rr;
-    const double sin_az = geo[2 * (pi * R + rr)];
-    const double mig = geo[2 * (pi * R + rr) + 1];
+    const double sin_az = geo[3 * (pi * R + rr)];
+    const double mig = geo[3 * (pi * R + rr) + 1];
+    const double phi = geo[3 * (pi * R + rr) + 2];
     float2 fch[OSAR_MAX_CH];
 #pragma unroll
     for (int ch = 0; ch < OSAR_MAX_CH; ++ch)
@@ -189,6 +313,28 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
           const float a0 = d.amp[ao];
           const float a1 = d.amp[ao + plane];
           scene[pol] = make_float2(a0 * b0.x + a1 * b1.x, a0 * b0.y + a1 * b1.y);
+        }
+      }
+      if (SPEC) {
+        // Esn = exp(-i 2 k0 sr_surface) (4 pi)^1.5 E   (sar_raw_nompi.py:405-426)
+        const double sxv = lerp_field(d.sx, so, plane, rk, w);
+        const double syv = lerp_field(d.sy, so, plane, rk, w);
+        const double sxxv = lerp_field(d.sxx, so, plane, rk, w);
+        const double syyv = lerp_field(d.syy, so, plane, rk, w);
+        const double sxyv = lerp_field(d.sxy, so, plane, rk, w);
+        zd e[2];
+        if (ka_spa(d.k0, spol, d.inc_col[x], phi, sxv, syv, sxxv, syyv, sxyv, d.facet_dx,
+                   d.facet_dy, e)) {
+          const double ph = -(2. * d.k0 * g.srs);
+          double ps, pcs;
+          sincos(ph, &ps, &pcs);
+          const double c15 = 44.546623974653535;  // (4 pi)^1.5
+          for (int pol = 0; pol < npol; ++pol) {
+            const zd esn = zscl(zmul({pcs, ps}, e[pol]), c15);
+            scene[pol].x += (float)esn.re;
+            scene[pol].y += (float)esn.im;
+            if (pol == npol - 1) spec_acc += esn.re * esn.re + esn.im * esn.im;
+          }
         }
       }
       const bool inwin = (dbin - 4 >= wb) && (dbin + 4 <= wend);
@@ -233,6 +379,12 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
       }
     }
   }
+  if (SPEC) {
+    // sum |Esn|^2 over the lanes of this pulse (lanes pi, pi + P, ...), one atomic per wave
+#pragma unroll
+    for (int off = P; off < 64; off <<= 1) spec_acc += __shfl_xor(spec_acc, off);
+    if (fsub == 0 && active && d.spec_energy) atomicAdd(&d.spec_energy[p], spec_acc);
+  }
   __syncthreads();
 
   // Add the LDS windows to the raw cube.
@@ -252,21 +404,28 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
   }
 }
 
-template <int P>
+template <int P, bool SPEC>
 int launch_accumulate(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipStream_t s) {
   const int nlines = d->n_pol * d->n_ch * P;
-  const size_t lds = sizeof(double) * 2 * P * R + sizeof(float2) * P * d->n_ch * R +
+  const size_t lds = sizeof(double) * 3 * P * R + sizeof(float2) * P * d->n_ch * R +
                      sizeof(float) * (size_t)nlines * 2 * W + sizeof(float) * 164 +
                      sizeof(int) * P;
   OSAR_REQUIRE(lds <= ctx->max_lds, OSAR_EUNSUPPORTED,
                "echo window needs %zu B of LDS (limit %zu); lower window_bins or "
                "pulses_per_block", lds, ctx->max_lds);
-  OSAR_HIP(hipFuncSetAttribute((const void*)echo_accumulate_kernel<P>,
+  OSAR_HIP(hipFuncSetAttribute((const void*)echo_accumulate_kernel<P, SPEC>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   dim3 grid((d->npulses + P - 1) / P, (d->ny + R - 1) / R);
-  hipLaunchKernelGGL(echo_accumulate_kernel<P>, grid, dim3(kThreads), lds, s, *d, W, R);
+  hipLaunchKernelGGL((echo_accumulate_kernel<P, SPEC>), grid, dim3(kThreads), lds, s, *d, W,
+                     R);
   OSAR_LAUNCHED("echo_accumulate_kernel");
   return OSAR_OK;
+}
+
+template <int P>
+int launch_accumulate_p(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipStream_t s) {
+  if (d->flags & OSAR_ECHO_SPECULAR) return launch_accumulate<P, true>(ctx, d, W, R, s);
+  return launch_accumulate<P, false>(ctx, d, W, R, s);
 }
 
 // chan_profile_numba: one thread per facet, taps added with global atomics.
@@ -332,6 +491,16 @@ extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void
     OSAR_PTR(d->cap_rate);
   }
   if (d->flags & OSAR_ECHO_XTI) OSAR_PTR(d->dinc_col);
+  if (d->flags & OSAR_ECHO_SPECULAR) {
+    OSAR_PTR(d->sx);
+    OSAR_PTR(d->sy);
+    OSAR_PTR(d->sxx);
+    OSAR_PTR(d->syy);
+    OSAR_PTR(d->sxy);
+    OSAR_PTR(d->inc_col);
+  }
+  OSAR_REQUIRE(d->pol >= 0 && d->pol <= 2 && (d->pol == 2) == (d->n_pol == 2), OSAR_EINVAL,
+               "pol %d inconsistent with n_pol %d", d->pol, d->n_pol);
   int P = d->pulses_per_block ? d->pulses_per_block : 8;
   const int R = d->rows_per_block ? d->rows_per_block : 64;
   int W = d->window_bins ? d->window_bins : 512;
@@ -340,10 +509,10 @@ extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void
   hipStream_t s = (hipStream_t)stream;
   OSAR_HIP(hipSetDevice(ctx->device));
   switch (P) {
-    case 1: return launch_accumulate<1>(ctx, d, W, R, s);
-    case 2: return launch_accumulate<2>(ctx, d, W, R, s);
-    case 4: return launch_accumulate<4>(ctx, d, W, R, s);
-    case 8: return launch_accumulate<8>(ctx, d, W, R, s);
+    case 1: return launch_accumulate_p<1>(ctx, d, W, R, s);
+    case 2: return launch_accumulate_p<2>(ctx, d, W, R, s);
+    case 4: return launch_accumulate_p<4>(ctx, d, W, R, s);
+    case 8: return launch_accumulate_p<8>(ctx, d, W, R, s);
     default: break;
   }
   osar_set_error("pulses_per_block must be 1, 2, 4 or 8 (got %d)", P);

@@ -21,6 +21,45 @@ def _t(a, dtype, device):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
 
 
+def synth_fields(ctx, surf, kx, ky, coefs, times, out, fields=None):
+    """osar_surface_synth: fields of surface `surf` (a SurfaceInit) at `times`.
+
+    kx, ky, coefs: device tensors (float32 [nx], [ny], complex64 [ny][nx]).
+    out[name]: float32 device tensor [>= n_times][nx][ny] (x-major) per field.
+    Returns the device tensor of times, which must stay alive until the
+    stream has run the launch.
+    """
+    import torch
+    fields = fields or list(out.keys())
+    if getattr(surf, 'swell_enable', False):
+        raise NotImplementedError('monochromatic swell is not supported on the GPU path')
+    times_t = torch.as_tensor(np.asarray(times, dtype=np.float64), device=coefs.device)
+    d = _lib.SurfaceDesc()
+    d.nx, d.ny = surf.Nx, surf.Ny
+    mask = 0
+    for n in fields:
+        mask |= _lib.FIELD_BITS[n]
+    d.fields = mask
+    d.choppy = 1 if surf.choppy else 0
+    d.n_times = len(times)
+    d.depth = -1.0 if surf.depth is None else float(surf.depth)
+    d.current_x, d.current_y = float(surf.current[0]), float(surf.current[1])
+    d.kx_res, d.ky_res = float(surf.kx_res), float(surf.ky_res)
+    d.kx, d.ky, d.coefs = _lib.ptr(kx), _lib.ptr(ky), _lib.ptr(coefs)
+    d.times = _lib.ptr(times_t)
+    for i, n in enumerate(_lib.FIELD_ORDER):
+        if n in fields:
+            t = out[n]
+            if t.dtype != torch.float32 or not t.is_contiguous():
+                raise TypeError('field %s must be a contiguous float32 tensor' % n)
+            if tuple(t.shape[-2:]) != (surf.Nx, surf.Ny) or t.shape[0] < len(times):
+                raise TypeError('field %s has shape %s, need [>=%d, %d, %d]'
+                                % (n, tuple(t.shape), len(times), surf.Nx, surf.Ny))
+            d.out[i] = t.data_ptr()
+    _lib.check(ctx.lib.osar_surface_synth(ctx.h, ctypes.byref(d), _lib.stream_ptr()))
+    return times_t
+
+
 class LaunchShape:
     """Echo-accumulator launch shape: pulses per workgroup, rows per workgroup, LDS window."""
 
@@ -136,10 +175,14 @@ class DeviceScene:
         d.rcs_sum = _lib.ptr(rcs_sum_out)
         _lib.check(self.ctx.lib.osar_facet_bragg(self.ctx.h, ctypes.byref(d), _lib.stream_ptr()))
 
-    def echo_desc(self, dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape=None):
+    def echo_desc(self, dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape=None,
+                  spec=None, spec_energy=None):
+        """Descriptor of one accumulator launch. spec: dict of slope/curvature snapshot
+        tensors (Diffx, Diffy, Diffxx, Diffyy, Diffxy) when specular is enabled."""
         sc = self.sc
         shape = shape or self.shape
         d = _lib.EchoDesc()
+        d.pol = POL_CODE.get(sc.pol, 0)
         d.nx, d.ny = sc.Nx, sc.Ny
         d.n_pulses_total = self.n_pulses
         d.pulse0, d.npulses = int(pulse0), int(npulses)
@@ -155,6 +198,16 @@ class DeviceScene:
             flags |= _lib.ECHO_BRAGG
         if np.any(sc.b_xti != 0):
             flags |= _lib.ECHO_XTI
+        if sc.spec_enable:
+            if spec is None:
+                raise ValueError('specular scattering needs the slope/curvature snapshots')
+            flags |= _lib.ECHO_SPECULAR
+            d.sx, d.sy = _lib.ptr(spec['Diffx']), _lib.ptr(spec['Diffy'])
+            d.sxx, d.syy = _lib.ptr(spec['Diffxx']), _lib.ptr(spec['Diffyy'])
+            d.sxy = _lib.ptr(spec['Diffxy'])
+            d.inc_col = _lib.ptr(self.inc_col)
+            d.facet_dx, d.facet_dy = float(sc.surface.dx), float(sc.surface.dy)
+            d.spec_energy = _lib.ptr(spec_energy)
         d.flags = flags
         d.pulses_per_block = shape.pulses_per_block
         d.rows_per_block = shape.rows_per_block
@@ -183,9 +236,11 @@ class DeviceScene:
             d.ind0_m, d.ind0_p = _lib.ptr(self.ind0_m), _lib.ptr(self.ind0_p)
         return d
 
-    def accumulate(self, dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape=None):
+    def accumulate(self, dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape=None,
+                   spec=None, spec_energy=None):
         """Add pulses [pulse0, pulse0+npulses) of every channel/pol into raw (device)."""
-        d = self.echo_desc(dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape)
+        d = self.echo_desc(dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape,
+                           spec, spec_energy)
         _lib.check(self.ctx.lib.osar_echo_accumulate(self.ctx.h, ctypes.byref(d),
                                                      _lib.stream_ptr()))
 
@@ -200,34 +255,8 @@ class DeviceScene:
 
     def synth(self, times, out, fields=None):
         """Surface fields at `times` into out[name] = float32 [n_times][nx][ny] (x-major)."""
-        import torch
-        sc = self.sc
-        surf = sc.surface
-        fields = fields or list(out.keys())
-        if getattr(surf, 'swell_enable', False):
-            raise NotImplementedError('monochromatic swell is not supported on the GPU path')
-        times_t = torch.as_tensor(np.asarray(times, dtype=np.float64), device=self.dev)
-        d = _lib.SurfaceDesc()
-        d.nx, d.ny = sc.Nx, sc.Ny
-        mask = 0
-        for n in fields:
-            mask |= _lib.FIELD_BITS[n]
-        d.fields = mask
-        d.choppy = 1 if surf.choppy else 0
-        d.n_times = len(times)
-        d.depth = -1.0 if surf.depth is None else float(surf.depth)
-        d.current_x, d.current_y = float(surf.current[0]), float(surf.current[1])
-        d.kx_res, d.ky_res = float(surf.kx_res), float(surf.ky_res)
-        d.kx, d.ky, d.coefs = _lib.ptr(self.kx), _lib.ptr(self.ky), _lib.ptr(self.coefs)
-        d.times = _lib.ptr(times_t)
-        for i, n in enumerate(_lib.FIELD_ORDER):
-            if n in fields:
-                t = out[n]
-                assert t.dtype == torch.float32 and t.is_contiguous()
-                assert tuple(t.shape[-2:]) == (sc.Nx, sc.Ny) and t.shape[0] >= len(times)
-                d.out[i] = t.data_ptr()
-        _lib.check(self.ctx.lib.osar_surface_synth(self.ctx.h, ctypes.byref(d), _lib.stream_ptr()))
-        self._keep = times_t   # alive until the stream has consumed it
+        self._keep = synth_fields(self.ctx, self.sc.surface, self.kx, self.ky, self.coefs,
+                                  times, out, fields)
 
     # ------------------------------------------------------------------
     def run(self, p0=0, p1=None, chunk_snaps=32, raw=None, shape=None, timer=None):
@@ -255,6 +284,9 @@ class DeviceScene:
         if sc.bragg_enable:
             amp = torch.empty((C + 2, self.npol, 2, nx, ny), dtype=torch.float32, device=self.dev)
         held = -1           # refresh index whose amplitudes sit in amp[0]
+        spec_energy = None
+        if sc.spec_enable:
+            spec_energy = torch.zeros(self.n_pulses, dtype=torch.float64, device=self.dev)
         p = p0
         while p < p1:
             m0 = p // K
@@ -290,7 +322,8 @@ class DeviceScene:
                 self._mark(timer, 'bragg', ev)
             ev = self._events(timer)
             self.accumulate(buf['Dx'], buf['Dy'], buf['Dz'], m0, amp, r0, p, q1 - p, raw,
-                            shape)
+                            shape, spec=buf if sc.spec_enable else None,
+                            spec_energy=spec_energy)
             self._mark(timer, 'echo', ev)
             if sc.bragg_enable:
                 # the last refresh in force stays in amp[0] for the next chunk
@@ -303,6 +336,10 @@ class DeviceScene:
             sums = rcs_sum.cpu().numpy()
             slots = sc.rcs_slot[p0:p1]
             nrcs[:, p0:p1] = sums[slots].T / sc.surface_area
+        if sc.spec_enable:
+            # specular |Esn|^2 of the last polarisation goes to every polarisation
+            # (sar_raw_nompi.py:427-428)
+            nrcs[:, p0:p1] += spec_energy[p0:p1].cpu().numpy()[None, :] / sc.surface_area
         return raw, nrcs
 
     @staticmethod

@@ -1,0 +1,111 @@
+"""NetCDF files of the raw-data path: RawFile and OceanStateFile.
+
+Same variable names, dimensions, units and the '_r'/'_i' convention for
+complex values as oceansar/ocs_io/raw.py:160-250, state.py:7-72 and
+netcdf.py:4-45, so downstream readers (sar_processor.py:71-77) find what they
+expect. netCDF4 is not a dependency: files are written by
+scipy.io.netcdf_file as NetCDF-3 64-bit-offset, which netCDF4.Dataset reads.
+NetCDF-3 has no 'u4', so the state file stores Nx/Ny as 'i4'.
+"""
+import time
+
+import numpy as np
+from scipy.io import netcdf_file
+
+
+class _Handler:
+    """get/set with a trailing '*' for complex variables (netcdf.py:20-45)."""
+
+    def close(self):
+        self._f.close()
+
+    def get(self, name):
+        v = self._f.variables
+        if name[-1] == '*':
+            re = v[name[:-1] + '_r'][:]
+            c = np.empty(re.shape, dtype=np.complex64)
+            c.real = re
+            c.imag = v[name[:-1] + '_i'][:]
+            return c
+        var = v[name]
+        return var.getValue() if var.shape == () else var[:].copy()
+
+    def set(self, name, value):
+        v = self._f.variables
+        if name[-1] == '*':
+            v[name[:-1] + '_r'][:] = np.real(value)
+            v[name[:-1] + '_i'][:] = np.imag(value)
+        elif v[name].shape == ():
+            # scalar variable (assignValue indexes a 0-d array with [:] and fails on numpy 2)
+            v[name].data[...] = value
+        else:
+            v[name][:] = value
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _var(f, name, typ, dims, units):
+    x = f.createVariable(name, typ, dims)
+    x.units = units
+    return x
+
+
+class RawFile(_Handler):
+    """Raw data file (raw.py:160-250): raw_data_r/_i f8 [pol, ch, az, rg]."""
+
+    def __init__(self, file_name, mode, raw_dim=None):
+        self._f = netcdf_file(file_name, mode, version=2, mmap=False)
+        if mode == 'w':
+            f = self._f
+            f.description = 'OCEANSAR SAR Raw Data File'
+            f.history = 'Created ' + time.ctime(time.time())
+            f.source = 'OCEANSAR Simulator (oceansar_amd)'
+            if not raw_dim:
+                raise ValueError('Raw data dimensions are needed when creating a new file!')
+            if len(raw_dim) != 4:
+                raise ValueError('raw_dim must be (pol, ch, az, rg)')
+            for n, s in zip(('pol_dim', 'ch_dim', 'az_dim', 'rg_dim'), raw_dim):
+                f.createDimension(n, int(s))
+            _var(f, 'num_ch', 'i4', (), '[]')
+            dims = ('pol_dim', 'ch_dim', 'az_dim', 'rg_dim')
+            _var(f, 'raw_data_r', 'f8', dims, '[]')
+            _var(f, 'raw_data_i', 'f8', dims, '[]')
+            for n, u in (('inc_angle', '[deg]'), ('f0', '[Hz]'), ('ant_L', '[m]'),
+                         ('prf', '[Hz]'), ('v_ground', '[m/s]'), ('orbit_alt', '[m]'),
+                         ('sr0', '[m]'), ('rg_sampling', '[Hz]'), ('rg_bw', '[Hz]')):
+                _var(f, n, 'f8', (), u)
+            _var(f, 'b_ati', 'f8', ('ch_dim',), '[m]')
+            _var(f, 'b_xti', 'f8', ('ch_dim',), '[m]')
+            _var(f, 'NRCS_avg', 'f8', ('pol_dim', 'az_dim'), '[]')
+
+
+class OceanStateFile(_Handler):
+    """Surface reuse file (state.py:7-72): grid, wind, swell and wave_coefs_r/_i."""
+
+    SCALARS = ('Lx', 'Ly', 'dx', 'dy', 'wind_dir', 'wind_dir_eff', 'wind_fetch', 'wind_U',
+               'wind_U_eff', 'current_mag', 'current_dir', 'swell_enable', 'swell_ampl',
+               'swell_dir', 'swell_wl', 'swell_ph0', 'choppy_enable')
+
+    def __init__(self, file_name, mode, ocean_dim=None):
+        self._f = netcdf_file(file_name, mode, version=2, mmap=False)
+        if mode == 'w':
+            f = self._f
+            f.description = 'OCEANSAR Library Ocean State File'
+            f.history = 'Created ' + time.ctime(time.time())
+            f.source = 'OCEANSAR Library (oceansar_amd)'
+            if not ocean_dim:
+                raise ValueError('Ocean dimensions are needed when creating new file!')
+            f.createDimension('y_dim', int(ocean_dim[0]))
+            f.createDimension('x_dim', int(ocean_dim[1]))
+            _var(f, 'Nx', 'i4', (), '[]')
+            _var(f, 'Ny', 'i4', (), '[]')
+            for n in self.SCALARS:
+                _var(f, n, 'f4', (), '[]')
+            _var(f, 'kx', 'f4', ('y_dim', 'x_dim'), '[rad/m]')
+            _var(f, 'ky', 'f4', ('y_dim', 'x_dim'), '[rad/m]')
+            _var(f, 'wave_coefs_r', 'f8', ('y_dim', 'x_dim'), '[]')
+            _var(f, 'wave_coefs_i', 'f8', ('y_dim', 'x_dim'), '[]')

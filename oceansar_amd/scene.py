@@ -56,7 +56,7 @@ class BraggModel:
 class SarScene:
     """All host-side inputs of one raw-data simulation."""
 
-    def __init__(self, cfg, seed=None, n_pulses=None):
+    def __init__(self, cfg, seed=None, n_pulses=None, surface=None):
         sar, ocean, srg = cfg.sar, cfg.ocean, cfg.srg
         sim = getattr(cfg, 'sim', None)
         if seed is None:
@@ -102,7 +102,7 @@ class SarScene:
             depth = None if depth.lower() == 'none' else float(depth)
         swell_dir_enable = bool(_get(ocean, 'swell_dir_enable', False))
         swell_enable = bool(_get(ocean, 'swell_enable', False))
-        self.surface = init_surface(
+        self.surface = surface if surface is not None else init_surface(
             float(ocean.Lx), float(ocean.Ly), float(ocean.dx), float(ocean.dy),
             str(ocean.spec_model), str(ocean.spread_model),
             np.deg2rad(ocean.wind_dir), float(ocean.wind_fetch), float(ocean.wind_U),

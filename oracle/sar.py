@@ -13,14 +13,17 @@ import numpy as np
 
 from . import bragg as _bragg
 from . import echo as _echo
+from . import ka as _ka
 from . import surface as _surface
 
 
 class SnapshotCache:
     """Surface snapshots (float32 fields) synthesised on demand at t = m*K*t_step."""
 
-    def __init__(self, sc, compute=('D', 'Diff')):
+    def __init__(self, sc, compute=None):
         self.sc = sc
+        if compute is None:
+            compute = ('D', 'Diff', 'Diff2') if sc.spec_enable else ('D', 'Diff')
         self.compute = compute
         self.cache = {}
 
@@ -82,8 +85,21 @@ def pulse_scene(sc, snaps, p, rcs):
     sr_surface = (sr - cos_inc * f['Dz'] + az / 2 * sin_az + f['Dx'] * sin_inc +
                   f['Dy'] * sin_az)
     wave_dinc = (f['Dz'] * sin_inc + f['Dx'] * sin_inc) / sc.sr0
-    scenes = {}
+    scenes = {pol: np.zeros((sc.Ny, sc.Nx), dtype=np.complex128) for pol in sc.pols}
     k0 = sc.k0
+    spec_energy = 0.
+    if sc.spec_enable:
+        # KA stationary-phase specular field (sar_raw_nompi.py:401-428)
+        g = lerp_fields(snaps, sc, p, ('Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy'))
+        _, _, az_proj = row_geometry(sc, p)
+        for pol in sc.pols:
+            E = _ka.field_spa(k0, pol, sc.inc, az_proj[:, 0], g['Diffx'], g['Diffy'],
+                              g['Diffxx'], g['Diffyy'], g['Diffxy'],
+                              float(sc.surface.dx), float(sc.surface.dy))
+            Esn = np.exp(-1j * (2. * k0 * sr_surface)) * (4. * np.pi) ** 1.5 * E
+            scenes[pol] = scenes[pol] + Esn
+            # the driver sums |Esn|^2 of the last polarisation computed (vv when present)
+            spec_energy = np.sum(np.abs(Esn) ** 2) / sc.surface_area
     if sc.bragg_enable:
         surf_phase = -(2 * k0) * sr_surface
         cap_phase = (2 * k0) * sc.t_step * sc.c_b.reshape(1, -1) * (p + 1)
@@ -93,17 +109,12 @@ def pulse_scene(sc, snaps, p, rcs):
         scats = np.array([sc_m, sc_p])
         for pol in sc.pols:
             amp = np.sqrt(rcs[pol])
-            scenes[pol] = np.sum(amp * np.exp(1j * ph) * scats, axis=0)
-    else:
-        for pol in sc.pols:
-            scenes[pol] = np.zeros((sc.Ny, sc.Nx), dtype=np.complex128)
-    return sr_surface, wave_dinc, sin_az, scenes
+            scenes[pol] = scenes[pol] + np.sum(amp * np.exp(1j * ph) * scats, axis=0)
+    return sr_surface, wave_dinc, sin_az, scenes, spec_energy
 
 
 def simulate(sc, p0=0, p1=None, snaps=None):
     """Raw cube for pulses [p0, p1) (un-calibrated, noise-free)."""
-    if sc.spec_enable:
-        raise NotImplementedError('specular scattering: use oracle.ka through simulate_ka')
     p1 = sc.az_steps if p1 is None else p1
     snaps = snaps or SnapshotCache(sc)
     npol = len(sc.pols)
@@ -114,10 +125,11 @@ def simulate(sc, p0=0, p1=None, snaps=None):
         if sc.bragg_enable and sc.rcs_slot[p] != rcs_slot:
             rcs_slot = sc.rcs_slot[p]
             rcs = bragg_rcs(sc, snaps, int(sc.rcs_refresh[rcs_slot]))
-        sr_surface, wave_dinc, sin_az, scenes = pulse_scene(sc, snaps, p, rcs)
+        sr_surface, wave_dinc, sin_az, scenes, spec_e = pulse_scene(sc, snaps, p, rcs)
         beam = beam_pattern(sc, sin_az)
         tot_dinc = (sc.inc.reshape(1, -1) - sc.inc_angle) + wave_dinc
         for ip, pol in enumerate(sc.pols):
+            nrcs[ip, p - p0] += spec_e
             if sc.bragg_enable:
                 nrcs[ip, p - p0] += np.sum(rcs[pol]) / sc.surface_area
             for ch in range(sc.num_ch):

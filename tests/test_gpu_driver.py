@@ -1,0 +1,59 @@
+"""GPU: the cfg-driven sar_raw() entry and the OceanSurface interface vs the oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_sar_raw_file_matches_oracle(need_gpu, tmp_path):
+    from oceansar_amd import configs, raw_io
+    from oceansar_amd.cfg import ConfigFile
+    from oceansar_amd.sar_raw import sar_raw
+    from oceansar_amd.scene import SarScene
+    from oceansar_amd.utils import db2lin
+    import oracle.sar as osar
+    text = configs.baseline(1, L=64., n_pulses=96, K=8, num_ch=2, b_ati=[0., 10.])
+    cfg_path = configs.write(str(tmp_path / 'sim.cfg'), text)
+    out = str(tmp_path / 'raw.nc')
+    ocean = str(tmp_path / 'ocean.nc')
+    timings = sar_raw(cfg_path, out, ocean, False, str(tmp_path / 'err.nc'), False)
+    assert timings['pulse_loop'] > 0
+    with raw_io.RawFile(out, 'r') as rf:
+        got = rf.get('raw_data_r') + 1j * rf.get('raw_data_i')
+        nrcs = rf.get('NRCS_avg')
+        assert rf.get('num_ch') == 2
+    sc = SarScene(ConfigFile(cfg_path))
+    ref, ref_nrcs = osar.simulate(sc)
+    ref = osar.post_process(sc, ref)
+    namp = db2lin(sc.nesz, amplitude=True) / np.sqrt(2.)
+    rng = sc.rng_noise
+    for ip in range(ref.shape[0]):
+        ref[ip] = ref[ip] + namp * (rng.standard_normal(ref[ip].shape) +
+                                    1j * rng.standard_normal(ref[ip].shape))
+    assert got.shape == ref.shape == (1, 2, 96, sc.rg_samp_orig)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err < 1e-4, err
+    np.testing.assert_allclose(nrcs, ref_nrcs, rtol=1e-9)
+    # reuse of the ocean state file gives the same surface
+    out2 = str(tmp_path / 'raw2.nc')
+    sar_raw(cfg_path, out2, ocean, True, str(tmp_path / 'err.nc'), False)
+    with raw_io.RawFile(out2, 'r') as rf:
+        got2 = rf.get('raw_data_r') + 1j * rf.get('raw_data_i')
+    assert np.linalg.norm(got2 - got) / np.linalg.norm(got) < 1e-5
+
+
+def test_ocean_surface_t_setter(need_gpu):
+    import oracle.surface as osurf
+    from oceansar_amd.ocean import OceanSurface
+    s = OceanSurface()
+    s.init(128., 128., 1., 1., 'auto', 'elfouhaily', 'elfouhaily', 0.3, 500e3, 9.,
+           0.2, 1.0, compute=['D', 'Diff', 'Diff2'], depth=50., seed=11)
+    s.t = 3.25
+    ref = osurf.synth(s.state, 3.25, ('D', 'Diff', 'Diff2'))
+    for n in ('Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy'):
+        got = getattr(s, n)
+        assert got.shape == (s.Ny, s.Nx) and got.dtype == np.float32
+        assert np.mean(got == ref[n]) > 0.999
+        scale = np.sqrt(np.mean(ref[n].astype(float) ** 2))
+        assert np.max(np.abs(got - ref[n])) < 1e-6 * scale
+    assert s.t == 3.25

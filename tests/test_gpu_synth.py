@@ -108,3 +108,24 @@ def test_end_to_end_dp_multichannel_xti(need_gpu, tmp_path_factory):
     print('DP/XTI rel RMS %.3e phase RMS %.3e' % (e, ph))
     assert e < 1e-4 and ph < 1e-3
     assert np.allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-9)
+
+
+@pytest.mark.parametrize('pol', ['vv', 'DP'])
+def test_end_to_end_specular(need_gpu, tmp_path_factory, pol):
+    """Bragg + Kirchhoff stationary-phase specular (workload-2 physics at 8 deg incidence,
+    where stationary-phase facets exist), two ATI channels."""
+    import torch
+    import oracle.sar as osar
+    from oceansar_amd.engine import DeviceScene
+    sc = _scene(tmp_path_factory, 'k' + pol, base=2, L=128., n_pulses=512, inc=8., pol=pol)
+    assert sc.spec_enable
+    ds = DeviceScene(sc)
+    p0, p1 = 250, 262
+    raw, nrcs = ds.run(p0, p1, chunk_snaps=1)
+    torch.cuda.synchronize()
+    got = raw[:, :, p0:p1].cpu().numpy().astype(np.complex128)
+    ref, ref_nrcs = osar.simulate(sc, p0, p1)
+    e, ph = rel_rms(got, ref), phase_rms(got, ref)
+    print('specular %s rel RMS %.3e phase RMS %.3e' % (pol, e, ph))
+    assert e < 1e-4 and ph < 1e-3
+    np.testing.assert_allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-9)

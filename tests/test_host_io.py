@@ -1,0 +1,47 @@
+"""RawFile / OceanStateFile layout and round trips (ocs_io/raw.py:160-250, state.py:7-72)."""
+import numpy as np
+from scipy.io import netcdf_file
+
+
+def test_rawfile_layout_and_round_trip(tmp_path):
+    from oceansar_amd import raw_io
+    rng = np.random.default_rng(0)
+    raw = rng.standard_normal((2, 3, 5, 7)) + 1j * rng.standard_normal((2, 3, 5, 7))
+    p = str(tmp_path / 'raw.nc')
+    with raw_io.RawFile(p, 'w', raw.shape) as rf:
+        rf.set('inc_angle', 35.)
+        rf.set('num_ch', 3)
+        rf.set('sr0', 616000.5)
+        rf.set('raw_data*', raw)
+        rf.set('NRCS_avg', np.arange(10.).reshape(2, 5))
+        rf.set('b_ati', np.array([0., 5., 10.]))
+    with netcdf_file(p, 'r', mmap=False) as f:
+        v = f.variables
+        assert v['raw_data_r'].dimensions == ('pol_dim', 'ch_dim', 'az_dim', 'rg_dim')
+        assert v['raw_data_r'].typecode() == 'd' and v['num_ch'].typecode() == 'i'
+        assert v['NRCS_avg'].dimensions == ('pol_dim', 'az_dim')
+        np.testing.assert_array_equal(v['raw_data_i'][:], raw.imag)
+        assert f.description.decode() == 'OCEANSAR SAR Raw Data File'
+    with raw_io.RawFile(p, 'r') as rf:
+        got = rf.get('raw_data*')
+        assert got.dtype == np.complex64         # get('x*') returns complex64 (netcdf.py:26)
+        np.testing.assert_allclose(got, raw.astype(np.complex64))
+        assert rf.get('sr0') == 616000.5
+
+
+def test_ocean_state_round_trip(tmp_path):
+    from oceansar_amd import configs
+    from oceansar_amd.ocean import OceanSurface
+    sc = configs.load_scene(configs.baseline(1, L=64.), tmp_path)
+    s = OceanSurface()
+    s._adopt(sc.surface, ['D'])
+    p = str(tmp_path / 'ocean.nc')
+    s.save(p)
+    s2 = OceanSurface()
+    s2.load(p, ['D', 'Diff'])
+    assert (s2.Nx, s2.Ny) == (sc.Nx, sc.Ny)
+    np.testing.assert_array_equal(s2.state.wave_coefs, sc.surface.wave_coefs)
+    np.testing.assert_array_equal(s2.state.kx_s, sc.surface.kx_s)
+    np.testing.assert_array_equal(s2.state.y, sc.surface.y)
+    assert s2.state.depth == sc.surface.depth
+    assert np.isclose(s2.state.wind_U_eff, sc.surface.wind_U_eff, rtol=1e-6)
