@@ -26,11 +26,13 @@
 // contraction off), so bin and sub-bin indices match an FP64 evaluation; the
 // phase is reduced to revolutions in FP64 before FP32 sin/cos; scene values,
 // channel factors and the accumulation are FP32.
+#include <limits.h>
+
 #include "common.h"
 
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
 constexpr int kTaps = 8;
 constexpr int kOvs = 20;
 constexpr int kTab = kTaps * kOvs + 1;  // 161
@@ -59,15 +61,6 @@ __device__ __forceinline__ Geo facet_range(double sr, double cosi, double sini, 
   g.srs = a;
   g.dinc = (Dz * sini + Dx * sini) / sr0;
   return g;
-}
-
-__device__ __forceinline__ double lerp_field(const float* f, size_t o0, size_t step, int rk,
-                                             double w) {
-#pragma clang fp contract(off)
-  double a = (double)f[o0];
-  if (rk == 0) return a;
-  double b = (double)f[o0 + step];
-  return (1. - w) * a + w * b;
 }
 
 // ---- Kirchhoff stationary-phase specular field (RCSKA.field, 'spa', monostatic;
@@ -188,65 +181,134 @@ __device__ bool ka_spa(double k0, int pol_code, double th, double ph, double Dx,
   return true;
 }
 
-template <int P, bool SPEC>
+// Per-lane inputs of one facet (one column x, the lane's row and pulse).
+template <int NPOL, bool SPEC>
+struct FacetIn {
+  float dx0, dy0, dz0, dx1, dy1, dz1;
+  float amp[NPOL][2];
+  osar_c32 sm, sp;
+  float s0[5], s1[5];  // specular slope / curvature fields at m and m+1
+};
+
+template <int NPOL, bool SPEC>
+__device__ __forceinline__ void load_facet(const osar_echo_desc& d, size_t idx, size_t so,
+                                           size_t plane, int rk, int p, int rslot, bool bragg,
+                                           FacetIn<NPOL, SPEC>& f) {
+  f.dx0 = d.dx[so];
+  f.dy0 = d.dy[so];
+  f.dz0 = d.dz[so];
+  if (rk != 0) {
+    f.dx1 = d.dx[so + plane];
+    f.dy1 = d.dy[so + plane];
+    f.dz1 = d.dz[so + plane];
+  }
+  if (bragg) {
+#pragma unroll
+    for (int pol = 0; pol < NPOL; ++pol) {
+      const size_t ao = ((size_t)(rslot * NPOL + pol) * 2) * plane + idx;
+      f.amp[pol][0] = d.amp[ao];
+      f.amp[pol][1] = d.amp[ao + plane];
+    }
+    f.sm = d.scat_m[d.ind0_m[idx] + p];
+    f.sp = d.scat_p[d.ind0_p[idx] + p];
+  }
+  if (SPEC) {
+    const float* src[5] = {d.sx, d.sy, d.sxx, d.syy, d.sxy};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      f.s0[k] = src[k][so];
+      if (rk != 0) f.s1[k] = src[k][so + plane];
+    }
+  }
+}
+
+__device__ __forceinline__ double lerp2(float a, float b, int rk, double w) {
+#pragma clang fp contract(off)
+  const double x = (double)a;
+  if (rk == 0) return x;
+  return (1. - w) * x + w * (double)b;
+}
+
+// Range-line layout in LDS: line (pol, ch, pulse) at offset line * LS doubles,
+// re in [0, W), im in [W, 2W); LS = 2W + pad keeps lines of different pulses
+// on different LDS banks. Lines are FP64: LDS FP64 add atomics run ~9x the
+// rate of FP32 add atomics on gfx950 and keep the partial sums exact enough.
+struct Lines {
+  double* acc;
+  int LS, W, wb, wend, nbins, npt;
+  osar_c32* raw;
+};
+
+// Add v to bin b of line (pol, ch) of this lane's pulse: LDS window when b is
+// inside it, global memory when only inside the range line, dropped outside
+// [0, nbins) (profile_integrator's edge rule, range_profile.py:40-45).
+template <int P, int NCH>
+__device__ __forceinline__ void deposit(const Lines& L, int pol, int ch, int nch, int pi, int p,
+                                        int b, float2 v) {
+  if (b >= L.wb && b < L.wend) {
+    double* lre = L.acc + (size_t)((pol * NCH + ch) * P + pi) * L.LS;
+    atomicAdd(&lre[b - L.wb], (double)v.x);
+    atomicAdd(&lre[L.W + b - L.wb], (double)v.y);
+  } else if (b >= 0 && b < L.nbins) {
+    osar_c32* o = L.raw + ((size_t)(pol * nch + ch) * L.npt + p) * L.nbins + b;
+    unsafeAtomicAdd(&o->re, v.x);
+    unsafeAtomicAdd(&o->im, v.y);
+  }
+}
+
+// Flush one channel-independent window slot: multiply by each channel's
+// per-(row, pulse) factor and deposit.
+template <int P, int NCH, int NPOL>
+__device__ __forceinline__ void flush_slot(const Lines& L, const float2 (&s)[NPOL],
+                                           const float2 (&fch)[NCH], int nch, int pi, int p,
+                                           int b) {
+#pragma unroll
+  for (int pol = 0; pol < NPOL; ++pol) {
+    if (s[pol].x == 0.f && s[pol].y == 0.f) continue;
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      if (ch >= nch) break;
+      deposit<P, NCH>(L, pol, ch, nch, pi, p, b, osar::cmul(s[pol], fch[ch]));
+    }
+  }
+}
+
+// XTI: the cross-track phase varies per facet, so channels cannot share a
+// window; this template variant deposits every tap of every channel.
+template <int P, int NCH, int NPOL, bool SPEC, bool XTI>
 __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_desc d, int W,
-                                                                   int R) {
+                                                                   int R, int LS) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  const int npol = d.n_pol, nch = d.n_ch;
-  const int nlines = npol * nch * P;
-  double* geo = (double*)lds_raw;                        // [P][R][3] sin_az, mig, az_proj
-  float2* F = (float2*)(geo + 3 * P * R);                // [P][nch][R]
-  float* acc = (float*)(F + P * nch * R);                // [nlines][2][W]
-  float* tab = acc + (size_t)nlines * 2 * W;             // [kTab]
-  int* wlo = (int*)(tab + 164);                          // [P]
+  constexpr int NL = NPOL * NCH * P;
+  double* acc = (double*)lds_raw;                  // [NL][LS]
+  float* tab = (float*)(acc + (size_t)NL * LS);    // [kTab]
+  int* wlo = (int*)(tab + 164);                    // [P]
 
   const int tid = threadIdx.x;
   const int pstart = d.pulse0 + blockIdx.x * P;
   const int pend = d.pulse0 + d.npulses;
   const int y0 = blockIdx.y * R;
   const int rows = min(R, d.ny - y0);
+  const int nch = d.n_ch;
 
-  for (int i = tid; i < nlines * 2 * W; i += kThreads) acc[i] = 0.f;
+  for (int i = tid; i < NL * LS; i += kThreads) acc[i] = 0.;
   for (int i = tid; i < kTab; i += kThreads) tab[i] = d.sinc_tab[i];
-  for (int i = tid; i < P * R; i += kThreads) {
-    const int pi = i / R, r = i - pi * R;
-    const int p = pstart + pi;
-    double sin_az = 0., mig = 0., az_proj = 0.;
-    double beam = 0.;
-    if (p < pend && r < rows) {
-      const double t_now = p * d.t_step;
-      const double az_now = (t_now - d.t_span / 2.) * d.v_ground;
-      const double az = d.y_row[y0 + r] - az_now;
-      sin_az = az / d.sr0;
-      mig = (az / 2) * sin_az;
-      if (SPEC) az_proj = asin(az / d.gr0);
-      beam = np_sinc(d.ltx_over_l * sin_az) * np_sinc(d.lrx_over_l * sin_az);
-    }
-    geo[3 * i] = sin_az;
-    geo[3 * i + 1] = mig;
-    geo[3 * i + 2] = az_proj;
-    for (int ch = 0; ch < nch; ++ch) {
-      const double ph = -(d.k0 * d.b_ati[ch]) * sin_az;
-      double s, c;
-      sincos(ph, &s, &c);
-      F[(pi * nch + ch) * R + r] = make_float2((float)(beam * c), (float)(beam * s));
-    }
-  }
-  __syncthreads();
   if (tid < P) {
-    double mmin = 1e300, mmax = -1e300;
-    for (int r = 0; r < rows; ++r) {
-      const double m = geo[3 * (tid * R + r) + 1];
-      mmin = fmin(mmin, m);
-      mmax = fmax(mmax, m);
-    }
+    // LDS window start for this pulse: smallest migration term over the tile's rows
+    // (monotonic in |az| on either side of az = 0)
+    const int p = min(pstart + tid, pend - 1);
+    const double az_now = (p * d.t_step - d.t_span / 2.) * d.v_ground;
+    const double a0 = d.y_row[y0] - az_now, a1 = d.y_row[y0 + rows - 1] - az_now;
+    double mmin = fmin((a0 / 2) * (a0 / d.sr0), (a1 / 2) * (a1 / d.sr0));
+    if (a0 <= 0. && a1 >= 0.) mmin = 0.;
     const double lo = (d.sr_col_min + mmin - d.resid_max - d.min_sr) / d.binsize;
     wlo[tid] = max((int)floor(lo) - 5, 0);
   }
   __syncthreads();
 
-  // Lane = (facet, pulse): FPW consecutive rows of one column x times P pulses.
-  // Fields are x-major (field[x][y]), so the FPW facets of a wave are contiguous.
+  // Lane = (facet row, pulse); the lane walks the columns x of its row, along
+  // which the range bin grows monotonically (by ~0.2 bin per column), so the
+  // lane's 8 taps land in a window of 8 consecutive bins held in registers.
   const int lane = tid & 63;
   const int pi = lane & (P - 1);
   const int fsub = lane / P;
@@ -261,121 +323,178 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
   const double w = (double)rk / (double)d.K;
   const size_t plane = (size_t)d.nx * d.ny;
   const size_t snap_off = (size_t)(m - d.snap0) * plane;
-  const int rslot = (d.flags & OSAR_ECHO_BRAGG) ? d.rcs_slot[pc] - d.rcs0 : 0;
-  const int wb = wlo[pi];
-  const int wend = min(wb + W, d.nbins);
-  const double two_k0 = 2 * d.k0;
   const bool bragg = (d.flags & OSAR_ECHO_BRAGG) != 0;
-  const bool xti = (d.flags & OSAR_ECHO_XTI) != 0;
-  // pol code of the specular field: 0 vv, 1 hh, 2 DP (e[0] hh, e[1] vv)
-  const int spol = npol == 2 ? 2 : (d.pol == 1 ? 1 : 0);
+  int rslot = bragg ? d.rcs_slot[pc] - d.rcs0 : 0;
+  const bool bragg_ok = bragg && rslot >= 0 && rslot < d.n_rcs_slots;
+  if (!bragg_ok) rslot = 0;
+  Lines L;
+  L.acc = acc;
+  L.LS = LS;
+  L.W = W;
+  L.wb = wlo[pi];
+  L.wend = min(L.wb + W, d.nbins);
+  L.nbins = d.nbins;
+  L.npt = d.n_pulses_total;
+  L.raw = d.raw;
+  const double two_k0 = 2 * d.k0;
+  const int spol = NPOL == 2 ? 2 : (d.pol == 1 ? 1 : 0);
   double spec_acc = 0.;
 
   for (int rg = 0; rg < rows; rg += FPB) {
     const int r = rg + wave * FPW + fsub;
-    const bool row_ok = active && r < rows;
-    const int rr = row_ok ? r : 0;
-    const int yrow = y0 + rr;
-    const double sin_az = geo[3 * (pi * R + rr)];
-    const double mig = geo[3 * (pi * R + rr) + 1];
-    const double phi = geo[3 * (pi * R + rr) + 2];
-    float2 fch[OSAR_MAX_CH];
+    if (!(active && r < rows)) continue;
+    const int yrow = y0 + r;
+    // per-row quantities of this lane's pulse (sar_raw_nompi.py:352-363, 485-493)
+    const double az_now = (p * d.t_step - d.t_span / 2.) * d.v_ground;
+    const double az = d.y_row[yrow] - az_now;
+    const double sin_az = az / d.sr0;
+    const double mig = (az / 2) * sin_az;
+    const double phi = SPEC ? asin(az / d.gr0) : 0.;
+    const double beam = np_sinc(d.ltx_over_l * sin_az) * np_sinc(d.lrx_over_l * sin_az);
+    float2 fch[NCH];
 #pragma unroll
-    for (int ch = 0; ch < OSAR_MAX_CH; ++ch)
-      fch[ch] = ch < nch ? F[(pi * nch + ch) * R + rr] : make_float2(0.f, 0.f);
+    for (int ch = 0; ch < NCH; ++ch) {
+      double s = 0., c = 0.;
+      if (ch < nch) sincos(-(d.k0 * d.b_ati[ch]) * sin_az, &s, &c);
+      fch[ch] = make_float2((float)(beam * c), (float)(beam * s));
+    }
+    // register window: win[pol][j] accumulates bin wbase + j
+    float2 win[NPOL][kTaps];
+#pragma unroll
+    for (int pol = 0; pol < NPOL; ++pol)
+#pragma unroll
+      for (int j = 0; j < kTaps; ++j) win[pol][j] = make_float2(0.f, 0.f);
+    int wbase = INT_MIN;
+
+    FacetIn<NPOL, SPEC> cur, nxt;
+    load_facet<NPOL, SPEC>(d, (size_t)yrow, snap_off + yrow, plane, rk, p, rslot, bragg_ok, cur);
     for (int x = 0; x < d.nx; ++x) {
-      if (!row_ok) continue;
       const size_t idx = (size_t)x * d.ny + yrow;
-      const size_t so = snap_off + idx;
-      const double Dx = lerp_field(d.dx, so, plane, rk, w);
-      const double Dy = lerp_field(d.dy, so, plane, rk, w);
-      const double Dz = lerp_field(d.dz, so, plane, rk, w);
+      if (x + 1 < d.nx)   // prefetch the next column while this one is processed
+        load_facet<NPOL, SPEC>(d, idx + d.ny, snap_off + idx + d.ny, plane, rk, p, rslot,
+                               bragg_ok, nxt);
+      const double Dx = lerp2(cur.dx0, cur.dx1, rk, w);
+      const double Dy = lerp2(cur.dy0, cur.dy1, rk, w);
+      const double Dz = lerp2(cur.dz0, cur.dz1, rk, w);
       const Geo g = facet_range(d.sr_col[x], d.cos_inc[x], d.sin_inc[x], mig, sin_az, Dx, Dy,
                                 Dz, d.sr0);
       const double xb = (g.srs - d.min_sr) / d.binsize;
-      if (!(fabs(xb) < 1.0e9)) continue;  // non-finite or absurd range: no valid tap
-      const double fl = floor(xb);
+      const bool finite = fabs(xb) < 1.0e9;   // non-finite or absurd range: no valid tap
+      const double fl = finite ? floor(xb) : 0.;
       const int dbin = (int)fl;
       const int sb = min(max((int)rint(20.0 * (1.0 - (xb - fl))), 0), kOvs);
 
-      float2 scene[2] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
-      if (bragg && rslot >= 0 && rslot < d.n_rcs_slots) {
+      float2 scene[NPOL];
+#pragma unroll
+      for (int pol = 0; pol < NPOL; ++pol) scene[pol] = make_float2(0.f, 0.f);
+      if (bragg_ok) {
         const double surf = -two_k0 * g.srs;
         const double cap = d.cap_rate[x] * (double)(p + 1);
         const float2 e0 = osar::cis_fp64(surf - cap);
         const float2 e1 = osar::cis_fp64(surf + cap);
-        const osar_c32 sm = d.scat_m[d.ind0_m[idx] + p];
-        const osar_c32 sp = d.scat_p[d.ind0_p[idx] + p];
-        const float2 b0 = osar::cmul(e0, make_float2(sm.re, sm.im));
-        const float2 b1 = osar::cmul(e1, make_float2(sp.re, sp.im));
-        for (int pol = 0; pol < npol; ++pol) {
-          const size_t ao = ((size_t)(rslot * npol + pol) * 2) * plane + idx;
-          const float a0 = d.amp[ao];
-          const float a1 = d.amp[ao + plane];
+        const float2 b0 = osar::cmul(e0, make_float2(cur.sm.re, cur.sm.im));
+        const float2 b1 = osar::cmul(e1, make_float2(cur.sp.re, cur.sp.im));
+#pragma unroll
+        for (int pol = 0; pol < NPOL; ++pol) {
+          const float a0 = cur.amp[pol][0], a1 = cur.amp[pol][1];
           scene[pol] = make_float2(a0 * b0.x + a1 * b1.x, a0 * b0.y + a1 * b1.y);
         }
       }
       if (SPEC) {
         // Esn = exp(-i 2 k0 sr_surface) (4 pi)^1.5 E   (sar_raw_nompi.py:405-426)
-        const double sxv = lerp_field(d.sx, so, plane, rk, w);
-        const double syv = lerp_field(d.sy, so, plane, rk, w);
-        const double sxxv = lerp_field(d.sxx, so, plane, rk, w);
-        const double syyv = lerp_field(d.syy, so, plane, rk, w);
-        const double sxyv = lerp_field(d.sxy, so, plane, rk, w);
         zd e[2];
-        if (ka_spa(d.k0, spol, d.inc_col[x], phi, sxv, syv, sxxv, syyv, sxyv, d.facet_dx,
-                   d.facet_dy, e)) {
-          const double ph = -(2. * d.k0 * g.srs);
+        if (finite && ka_spa(d.k0, spol, d.inc_col[x], phi, lerp2(cur.s0[0], cur.s1[0], rk, w),
+                             lerp2(cur.s0[1], cur.s1[1], rk, w),
+                             lerp2(cur.s0[2], cur.s1[2], rk, w),
+                             lerp2(cur.s0[3], cur.s1[3], rk, w),
+                             lerp2(cur.s0[4], cur.s1[4], rk, w), d.facet_dx, d.facet_dy, e)) {
           double ps, pcs;
-          sincos(ph, &ps, &pcs);
+          sincos(-(2. * d.k0 * g.srs), &ps, &pcs);
           const double c15 = 44.546623974653535;  // (4 pi)^1.5
-          for (int pol = 0; pol < npol; ++pol) {
+#pragma unroll
+          for (int pol = 0; pol < NPOL; ++pol) {
             const zd esn = zscl(zmul({pcs, ps}, e[pol]), c15);
             scene[pol].x += (float)esn.re;
             scene[pol].y += (float)esn.im;
-            if (pol == npol - 1) spec_acc += esn.re * esn.re + esn.im * esn.im;
+            if (pol == NPOL - 1) spec_acc += esn.re * esn.re + esn.im * esn.im;
           }
         }
       }
-      const bool inwin = (dbin - 4 >= wb) && (dbin + 4 <= wend);
-      const int base = dbin - 4 - wb;
+      if (finite) {
+        if (!XTI) {
+          // slide the window so that it starts at bin dbin - 4
+          const int want = dbin - 4;
+          if (want != wbase) {
+            const int shift = (wbase == INT_MIN) ? -1 : want - wbase;
+            if (shift < 0 || shift > kTaps) {
+              if (wbase != INT_MIN) {
 #pragma unroll
-      for (int ch = 0; ch < OSAR_MAX_CH; ++ch) {
-        if (ch >= nch) break;
-        float2 f = fch[ch];
-        if (xti && d.b_xti[ch] != 0.0) {
-          const double tot = d.dinc_col[x] + g.dinc;
-          f = osar::cmul(f, osar::cis_fp64(-(d.k0 * d.b_xti[ch]) * tot));
-        }
-        for (int pol = 0; pol < npol; ++pol) {
-          const float2 v = osar::cmul(scene[pol], f);
-          float* lre = acc + (size_t)(((pol * nch + ch) * P + pi) * 2) * W;
-          float* lim = lre + W;
-          if (inwin) {
+                for (int j = 0; j < kTaps; ++j) {
+                  float2 s[NPOL];
 #pragma unroll
-            for (int j = 0; j < kTaps; ++j) {
-              const int jj = (j + fsub) & (kTaps - 1);
-              const float t = tab[sb + kOvs * jj];
-              atomicAdd(&lre[base + jj], t * v.x);
-              atomicAdd(&lim[base + jj], t * v.y);
+                  for (int pol = 0; pol < NPOL; ++pol) s[pol] = win[pol][j];
+                  flush_slot<P, NCH, NPOL>(L, s, fch, nch, pi, p, wbase + j);
+                }
+              }
+#pragma unroll
+              for (int pol = 0; pol < NPOL; ++pol)
+#pragma unroll
+                for (int j = 0; j < kTaps; ++j) win[pol][j] = make_float2(0.f, 0.f);
+            } else {
+              for (int k = 0; k < shift; ++k) {
+                float2 s[NPOL];
+#pragma unroll
+                for (int pol = 0; pol < NPOL; ++pol) s[pol] = win[pol][0];
+                flush_slot<P, NCH, NPOL>(L, s, fch, nch, pi, p, wbase + k);
+#pragma unroll
+                for (int pol = 0; pol < NPOL; ++pol) {
+#pragma unroll
+                  for (int j = 0; j < kTaps - 1; ++j) win[pol][j] = win[pol][j + 1];
+                  win[pol][kTaps - 1] = make_float2(0.f, 0.f);
+                }
+              }
             }
-          } else {
-            for (int j = 0; j < kTaps; ++j) {
-              const int b = dbin - 4 + j;
-              if (b < 0 || b >= d.nbins) continue;
-              const float t = tab[sb + kOvs * j];
-              if (b >= wb && b < wend) {
-                atomicAdd(&lre[b - wb], t * v.x);
-                atomicAdd(&lim[b - wb], t * v.y);
-              } else {
-                osar_c32* o = d.raw + ((size_t)(pol * nch + ch) * d.n_pulses_total + p) *
-                                          d.nbins + b;
-                unsafeAtomicAdd(&o->re, t * v.x);
-                unsafeAtomicAdd(&o->im, t * v.y);
+            wbase = want;
+          }
+#pragma unroll
+          for (int j = 0; j < kTaps; ++j) {
+            const float t = tab[sb + kOvs * j];
+#pragma unroll
+            for (int pol = 0; pol < NPOL; ++pol) {
+              win[pol][j].x = fmaf(t, scene[pol].x, win[pol][j].x);
+              win[pol][j].y = fmaf(t, scene[pol].y, win[pol][j].y);
+            }
+          }
+        } else {
+#pragma unroll
+          for (int ch = 0; ch < NCH; ++ch) {
+            if (ch >= nch) break;
+            float2 f = fch[ch];
+            if (d.b_xti[ch] != 0.0) {
+              const double tot = d.dinc_col[x] + g.dinc;
+              f = osar::cmul(f, osar::cis_fp64(-(d.k0 * d.b_xti[ch]) * tot));
+            }
+#pragma unroll
+            for (int pol = 0; pol < NPOL; ++pol) {
+              const float2 v = osar::cmul(scene[pol], f);
+              for (int j = 0; j < kTaps; ++j) {
+                const float t = tab[sb + kOvs * j];
+                deposit<P, NCH>(L, pol, ch, nch, pi, p, dbin - 4 + j,
+                                make_float2(t * v.x, t * v.y));
               }
             }
           }
         }
+      }
+      cur = nxt;
+    }
+    if (!XTI && wbase != INT_MIN) {
+#pragma unroll
+      for (int j = 0; j < kTaps; ++j) {
+        float2 s[NPOL];
+#pragma unroll
+        for (int pol = 0; pol < NPOL; ++pol) s[pol] = win[pol][j];
+        flush_slot<P, NCH, NPOL>(L, s, fch, nch, pi, p, wbase + j);
       }
     }
   }
@@ -388,46 +507,74 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
   __syncthreads();
 
   // Add the LDS windows to the raw cube.
-  for (int i = tid; i < nlines * W; i += kThreads) {
+  for (int i = tid; i < NL * W; i += kThreads) {
     const int line = i / W, b = i - line * W;
     const int lp = line % P;
-    const int pc2 = line / P;  // pol * nch + ch
+    const int pc2 = line / P;  // pol * NCH + ch
+    const int pol = pc2 / NCH, ch = pc2 - pol * NCH;
     const int pp = pstart + lp;
     const int gb = wlo[lp] + b;
-    if (pp >= pend || gb >= d.nbins) continue;
-    const float re = acc[(size_t)line * 2 * W + b];
-    const float im = acc[(size_t)line * 2 * W + W + b];
+    if (ch >= nch || pp >= pend || gb >= d.nbins) continue;
+    const float re = (float)acc[(size_t)line * LS + b];
+    const float im = (float)acc[(size_t)line * LS + W + b];
     if (re == 0.f && im == 0.f) continue;
-    osar_c32* o = d.raw + ((size_t)pc2 * d.n_pulses_total + pp) * d.nbins + gb;
+    osar_c32* o = d.raw + ((size_t)(pol * nch + ch) * d.n_pulses_total + pp) * d.nbins + gb;
     unsafeAtomicAdd(&o->re, re);
     unsafeAtomicAdd(&o->im, im);
   }
 }
 
-template <int P, bool SPEC>
+size_t echo_lds_bytes(int P, int nch_c, int npol, int LS) {
+  return sizeof(double) * (size_t)npol * nch_c * P * LS + sizeof(float) * 164 + sizeof(int) * P;
+}
+
+int line_stride(int W) {
+  // 2W rounded up to a multiple of 32 doubles, plus 4: consecutive lines start
+  // 8 banks apart
+  return ((2 * W + 31) / 32) * 32 + 4;
+}
+
+template <int P, int NCH, int NPOL, bool SPEC, bool XTI>
 int launch_accumulate(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipStream_t s) {
-  const int nlines = d->n_pol * d->n_ch * P;
-  const size_t lds = sizeof(double) * 3 * P * R + sizeof(float2) * P * d->n_ch * R +
-                     sizeof(float) * (size_t)nlines * 2 * W + sizeof(float) * 164 +
-                     sizeof(int) * P;
+  const int LS = line_stride(W);
+  const size_t lds = echo_lds_bytes(P, NCH, NPOL, LS);
   OSAR_REQUIRE(lds <= ctx->max_lds, OSAR_EUNSUPPORTED,
                "echo window needs %zu B of LDS (limit %zu); lower window_bins or "
                "pulses_per_block", lds, ctx->max_lds);
-  OSAR_HIP(hipFuncSetAttribute((const void*)echo_accumulate_kernel<P, SPEC>,
+  OSAR_HIP(hipFuncSetAttribute((const void*)echo_accumulate_kernel<P, NCH, NPOL, SPEC, XTI>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   dim3 grid((d->npulses + P - 1) / P, (d->ny + R - 1) / R);
-  hipLaunchKernelGGL((echo_accumulate_kernel<P, SPEC>), grid, dim3(kThreads), lds, s, *d, W,
-                     R);
+  hipLaunchKernelGGL((echo_accumulate_kernel<P, NCH, NPOL, SPEC, XTI>), grid, dim3(kThreads),
+                     lds, s, *d, W, R, LS);
   OSAR_LAUNCHED("echo_accumulate_kernel");
   return OSAR_OK;
 }
 
-template <int P>
-int launch_accumulate_p(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipStream_t s) {
-  if (d->flags & OSAR_ECHO_SPECULAR) return launch_accumulate<P, true>(ctx, d, W, R, s);
-  return launch_accumulate<P, false>(ctx, d, W, R, s);
+template <int P, int NCH, int NPOL, bool SPEC>
+int launch_x(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipStream_t s) {
+  if (d->flags & OSAR_ECHO_XTI) return launch_accumulate<P, NCH, NPOL, SPEC, true>(ctx, d, W, R, s);
+  return launch_accumulate<P, NCH, NPOL, SPEC, false>(ctx, d, W, R, s);
 }
 
+template <int P, int NCH, int NPOL>
+int launch_s(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipStream_t s) {
+  if (d->flags & OSAR_ECHO_SPECULAR) return launch_x<P, NCH, NPOL, true>(ctx, d, W, R, s);
+  return launch_x<P, NCH, NPOL, false>(ctx, d, W, R, s);
+}
+
+template <int P, int NCH>
+int launch_pol(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipStream_t s) {
+  if (d->n_pol == 2) return launch_s<P, NCH, 2>(ctx, d, W, R, s);
+  return launch_s<P, NCH, 1>(ctx, d, W, R, s);
+}
+
+template <int P>
+int launch_ch(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipStream_t s) {
+  if (d->n_ch <= 1) return launch_pol<P, 1>(ctx, d, W, R, s);
+  if (d->n_ch <= 2) return launch_pol<P, 2>(ctx, d, W, R, s);
+  if (d->n_ch <= 4) return launch_pol<P, 4>(ctx, d, W, R, s);
+  return launch_pol<P, 8>(ctx, d, W, R, s);
+}
 // chan_profile_numba: one thread per facet, taps added with global atomics.
 __global__ void echo_deposit_kernel(const double* srg, const osar_c32* scene, int64_t n,
                                     double binsize, double min_sr, const float* tab, int nsamp,
@@ -509,13 +656,12 @@ extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void
   hipStream_t s = (hipStream_t)stream;
   OSAR_HIP(hipSetDevice(ctx->device));
   switch (P) {
-    case 1: return launch_accumulate_p<1>(ctx, d, W, R, s);
-    case 2: return launch_accumulate_p<2>(ctx, d, W, R, s);
-    case 4: return launch_accumulate_p<4>(ctx, d, W, R, s);
-    case 8: return launch_accumulate_p<8>(ctx, d, W, R, s);
+    case 2: return launch_ch<2>(ctx, d, W, R, s);
+    case 4: return launch_ch<4>(ctx, d, W, R, s);
+    case 8: return launch_ch<8>(ctx, d, W, R, s);
     default: break;
   }
-  osar_set_error("pulses_per_block must be 1, 2, 4 or 8 (got %d)", P);
+  osar_set_error("pulses_per_block must be 2, 4 or 8 (got %d)", P);
   return OSAR_EINVAL;
 }
 

@@ -74,7 +74,7 @@ class LaunchShape:
 
 
 class DeviceScene:
-    def __init__(self, sc, device=0, lds_budget=80 * 1024):
+    def __init__(self, sc, device=0, lds_budget=160 * 1024):
         import torch
         self.sc = sc
         self.ctx = _lib.context(device)
@@ -114,7 +114,8 @@ class DeviceScene:
         return rs.s[np.arange(n) % rs.N].astype(np.complex64)
 
     def _resid_bound(self):
-        """Window margin for |residual range|: 6 sigma of |Dz|+|Dx|+|Dy| from the spectrum.
+        """Window margin for |residual range|: 8 x the spectral RMS elevation (|Dz| + |Dx|
+        rarely exceed ~5 RMS together), plus the current drift.
 
         Only sizes the LDS window; facets beyond it take the global-memory path.
         """
@@ -122,22 +123,33 @@ class DeviceScene:
         n2 = float(self.sc.Nx * self.sc.Ny)
         rms = math.sqrt(float(np.sum(np.abs(c) ** 2)) / 2.) / n2
         cur = float(np.hypot(*self.sc.surface.current)) * self.n_pulses * self.sc.t_step
-        return 6. * 3. * rms + cur + 0.5
+        return 8. * rms + cur + 0.5
 
-    def default_shape(self, lds_budget=80 * 1024, rows=64):
+    def window_bins(self, rows):
+        """LDS window covering a tile of `rows` rows at any pulse (see echo.hip)."""
         sc = self.sc
-        R = min(rows, sc.Ny)
         az_max = max(abs(sc.az_now(0)), abs(sc.az_now(sc.az_steps))) + sc.surface.Ly
-        mig_spread = R * float(sc.surface.dy) * az_max / sc.sr0
+        mig_spread = rows * float(sc.surface.dy) * az_max / sc.sr0
         span = (sc.sr.max() - sc.sr.min() + mig_spread + 2 * self.resid_max) / sc.binsize
-        W = int(math.ceil(span)) + 8 + 4
-        W = min(((W + 31) // 32) * 32, ((sc.rg_samp + 31) // 32) * 32)
-        lines = self.npol * sc.num_ch
-        for P in (8, 4, 2, 1):
-            lds = 16 * P * R + 8 * P * sc.num_ch * R + 8 * lines * P * W + 700
-            if lds <= lds_budget:
+        W = int(math.ceil(span)) + 8 + 6
+        return min(((W + 15) // 16) * 16, ((sc.rg_samp + 15) // 16) * 16)
+
+    @staticmethod
+    def lds_bytes(P, n_ch, npol, W):
+        """LDS of one accumulator workgroup (echo.hip: echo_lds_bytes, line_stride)."""
+        nch_c = 1 if n_ch <= 1 else 2 if n_ch <= 2 else 4 if n_ch <= 4 else 8
+        LS = ((2 * W + 31) // 32) * 32 + 4            # FP64 lines
+        return 8 * npol * nch_c * P * LS + 4 * 164 + 4 * P
+
+    def default_shape(self, lds_budget=160 * 1024):
+        """Largest P (pulses per workgroup) whose LDS lines fit; R = rows one pass covers."""
+        sc = self.sc
+        for P in (8, 4, 2):
+            R = min(8 * (64 // P), sc.Ny)      # 8 waves x 64/P rows
+            W = self.window_bins(R)
+            if self.lds_bytes(P, sc.num_ch, self.npol, W) <= lds_budget:
                 return LaunchShape(P, R, W)
-        return LaunchShape(1, R, W)
+        raise RuntimeError('range window of %d bins does not fit the LDS even at P=2' % W)
 
     # ------------------------------------------------------------------
     def new_raw(self):

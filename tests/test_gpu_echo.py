@@ -110,7 +110,8 @@ def test_accumulator_launch_shapes_agree(need_gpu, cfg1_scene):
         pr = int(sc.rcs_refresh[r])
         ds.bragg(dev['Diffx'][pr // sc.K - m0], dev['Diffy'][pr // sc.K - m0], pr, amp[r - r0])
     outs = []
-    for shape in (ds.shape, LaunchShape(1, 7, 64), LaunchShape(4, 256, 32), LaunchShape(2, 16, 128)):
+    for shape in (ds.shape, LaunchShape(2, 7, 64), LaunchShape(4, 256, 32), LaunchShape(2, 16, 128),
+                  LaunchShape(8, 100, 96)):
         raw = ds.new_raw()
         ds.accumulate(dev['Dx'], dev['Dy'], dev['Dz'], m0, amp, r0, p0, n, raw, shape=shape)
         outs.append(raw[:, :, p0:p0 + n].cpu().numpy().astype(np.complex128))
