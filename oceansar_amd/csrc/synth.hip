@@ -13,12 +13,13 @@
 // divides in the epilogue.
 //
 // Pass 1 (rows): one workgroup per row pair (j, -j): phased coefficients of
-// both rows in LDS, packed spectra built there, radix-4/2 Stockham FFT along x
-// in LDS, rows written to an FP64 workspace.
+// both rows in registers (each thread owns columns and their mirrors), packed
+// spectra built into LDS, radix-4/2 Stockham FFT along x in LDS, rows written
+// to an FP64 workspace in column-blocked layout.
 // Pass 2 (columns): one workgroup per (packed spectrum, block of CB columns):
-// CB columns staged in LDS, FFT along y, scaled by 1/(Nx Ny) and written as
-// float32 fields in x-major layout (field[x][y]) so that every store is a
-// contiguous run along y.
+// the block is read as one contiguous run, FFT along y in LDS, scaled by
+// 1/(Nx Ny) and written as float32 fields in x-major layout (field[x][y]) so
+// that every store is a contiguous run along y.
 #include <math.h>
 
 #include <vector>
@@ -69,30 +70,30 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-__device__ __forceinline__ double2 mult(int m, double kx, double ky, double thr) {
-  const double k = sqrt(kx * kx + ky * ky);
-  const bool good = k > thr;
+// Spectral multiplier m(k) for wave vector (kx, ky) with |k| and 1/|k| given
+// (1/|k| = 0 and k/|k| = 0 off the good-k set, as ocean.py:141-148 does).
+__device__ __forceinline__ double2 mult(int m, double kx, double ky, double ik) {
   switch (m) {
     case M_ONE: return make_double2(1., 0.);
-    case M_IKXN: return make_double2(0., good ? kx / k : 0.);
-    case M_IKYN: return make_double2(0., good ? ky / k : 0.);
+    case M_IKXN: return make_double2(0., kx * ik);
+    case M_IKYN: return make_double2(0., ky * ik);
     case M_IKX: return make_double2(0., kx);
     case M_IKY: return make_double2(0., ky);
     case M_NKX2: return make_double2(-(kx * kx), 0.);
     case M_NKY2: return make_double2(-(ky * ky), 0.);
     case M_NKXKY: return make_double2(-kx * ky, 0.);
-    case M_NKX2KI: return make_double2(good ? -(kx * kx) * (1. / k) : 0., 0.);
-    case M_NKY2KI: return make_double2(good ? -(ky * ky) * (1. / k) : 0., 0.);
+    case M_NKX2KI: return make_double2(-(kx * kx) * ik, 0.);
+    case M_NKY2KI: return make_double2(-(ky * ky) * ik, 0.);
     default: return make_double2(0., 0.);
   }
 }
 
 // H(m a)[k] = (m(k) a(k) + conj(m(-k) a(-k))) / 2
-__device__ __forceinline__ double2 herm(int m, double kx, double ky, double kxm, double kym,
-                                        double2 a, double2 am, double thr) {
+__device__ __forceinline__ double2 herm(int m, double kx, double ky, double ik, double kxm,
+                                        double kym, double ikm, double2 a, double2 am) {
   if (m == M_NONE) return make_double2(0., 0.);
-  const double2 g = cmul2(mult(m, kx, ky, thr), a);
-  const double2 gm = cmul2(mult(m, kxm, kym, thr), am);
+  const double2 g = cmul2(mult(m, kx, ky, ik), a);
+  const double2 gm = cmul2(mult(m, kxm, kym, ikm), am);
   return make_double2(0.5 * (g.x + gm.x), 0.5 * (g.y - gm.y));
 }
 
@@ -103,18 +104,20 @@ template <int NT, int BPT>
 __device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int ss, int es,
                          const double2* __restrict__ tw) {
   const int tid = threadIdx.x;
-  int Ns = 1;
+  int Ns = 1, logNs = 0;
   int left = logN;
   while (left > 0) {
-    const int R = left >= 2 ? 4 : 2;
-    const int nb = N / R;
-    const int total = nseq * nb;
+    const int lr = left >= 2 ? 2 : 1;  // log2 of the radix
+    const int R = 1 << lr;
+    const int lnb = logN - lr;          // butterflies per sequence = N / R = 1 << lnb
+    const int nb = 1 << lnb;
+    const int total = nseq << lnb;
     double2 v[BPT][4];
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
       const int id = tid + b * NT;
       if (id < total) {
-        const int s = id / nb, j = id - s * nb;
+        const int s = id >> lnb, j = id & (nb - 1);
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           if (r < R) v[b][r] = buf[s * ss + (j + r * nb) * es];
@@ -125,9 +128,9 @@ __device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int ss, int es
     for (int b = 0; b < BPT; ++b) {
       const int id = tid + b * NT;
       if (id < total) {
-        const int s = id / nb, j = id - s * nb;
+        const int s = id >> lnb, j = id & (nb - 1);
         const int jm = j & (Ns - 1);
-        const int step = jm * (N / (Ns * R));
+        const int step = jm << (logN - logNs - lr);   // jm * N / (Ns * R)
         if (R == 4) {
           const double2 x0 = v[b][0];
           const double2 x1 = cmul2(v[b][1], tw[step]);
@@ -153,18 +156,25 @@ __device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int ss, int es
       }
     }
     __syncthreads();
-    Ns *= R;
-    left -= (R == 4 ? 2 : 1);
+    Ns <<= lr;
+    logNs += lr;
+    left -= lr;
   }
 }
 
-constexpr int kRowThreads = 256;
+constexpr int kRowThreads = 512;
 constexpr int kColThreads = 512;
+constexpr int kRowPairs = kMaxN / kRowThreads;   // column pairs a row thread owns
 
+// Pass 1: one workgroup per row pair (j, -j) and snapshot. Thread t owns the
+// columns i = t, t + kRowThreads, ... of row j together with their mirrors -i of row
+// -j, so a(k) and a(-k) of every packed spectrum it builds are in its own
+// registers; only the FFT buffer lives in LDS (2 rows, 64 KB at 2048).
+// Rows go to the workspace in column-blocked layout W[q][x / cb][y][x % cb],
+// so that pass 2 reads each block of cb columns as one contiguous run.
 __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  double2* A = (double2*)lds_raw;  // [2][nx] phased coefficients of rows j, jm
-  double2* S = A + 2 * a.nx;       // [2][nx] packed spectrum
+  double2* S = (double2*)lds_raw;  // [2][nx] packed spectrum of rows j, jm
   const int j = blockIdx.x;
   const int jm = (a.ny - j) % a.ny;
   const int nrow = (j == jm) ? 1 : 2;
@@ -172,59 +182,80 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
   const double t = a.times[ti];
   const int tid = threadIdx.x;
   const int nx = a.nx;
-
-  for (int e = tid; e < nrow * nx; e += kRowThreads) {
-    const int r = e / nx, i = e - r * nx;
-    const int row = r == 0 ? j : jm;
-    const double kx = (double)a.kx[i], ky = (double)a.ky[row];
-    const double k = sqrt(kx * kx + ky * ky);
-    const double om = a.depth > 0. ? sqrt(kG * k * tanh(k * a.depth)) : sqrt(kG * k);
-    double sn, cs;
-    sincos(-(om * t), &sn, &cs);
-    const osar_c32 c = a.coefs[(size_t)row * nx + i];
-    A[r * nx + i] = cmul2(make_double2((double)c.re, (double)c.im), make_double2(cs, sn));
+  const int cb = a.cb;
+  const double kyj = (double)a.ky[j], kyjm = (double)a.ky[jm];
+  double2 A0[kRowPairs], A1[kRowPairs];   // a(k) at (j, i) and a(-k) at (jm, -i)
+  double ik0[kRowPairs], ik1[kRowPairs];  // 1/|k| (0 off the good-k set)
+#pragma unroll
+  for (int u = 0; u < kRowPairs; ++u) {
+    const int i = tid + u * kRowThreads;
+    if (i >= nx) break;
+    const int im = (nx - i) & (nx - 1);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int row = side == 0 ? j : jm, col = side == 0 ? i : im;
+      const double kx = (double)a.kx[col], ky = side == 0 ? kyj : kyjm;
+      const double k = sqrt(kx * kx + ky * ky);
+      const double om = a.depth > 0. ? sqrt(kG * k * tanh(k * a.depth)) : sqrt(kG * k);
+      double sn, cs;
+      sincos(-(om * t), &sn, &cs);
+      const osar_c32 c = a.coefs[(size_t)row * nx + col];
+      const double2 v = cmul2(make_double2((double)c.re, (double)c.im), make_double2(cs, sn));
+      const double ik = k > a.thr ? 1. / k : 0.;
+      if (side == 0) {
+        A0[u] = v;
+        ik0[u] = ik;
+      } else {
+        A1[u] = v;
+        ik1[u] = ik;
+      }
+    }
   }
-  __syncthreads();
   for (int qi = 0; qi < a.nq; ++qi) {
     const QPlan q = a.q[qi];
-    for (int e = tid; e < nrow * nx; e += kRowThreads) {
-      const int r = e / nx, i = e - r * nx;
-      const int rm = (nrow == 1) ? 0 : 1 - r;
-      const int row = r == 0 ? j : jm, rowm = r == 0 ? jm : j;
+#pragma unroll
+    for (int u = 0; u < kRowPairs; ++u) {
+      const int i = tid + u * kRowThreads;
+      if (i >= nx) break;
       const int im = (nx - i) & (nx - 1);
-      const double kx = (double)a.kx[i], ky = (double)a.ky[row];
-      const double kxm = (double)a.kx[im], kym = (double)a.ky[rowm];
-      const double2 ak = A[r * nx + i], am = A[rm * nx + im];
-      const double2 ha = herm(q.ma, kx, ky, kxm, kym, ak, am, a.thr);
-      const double2 hb = herm(q.mb, kx, ky, kxm, kym, ak, am, a.thr);
-      S[r * nx + i] = make_double2(ha.x - hb.y, ha.y + hb.x);  // ha + i hb
+      const double kx = (double)a.kx[i], kxm = (double)a.kx[im];
+      // S at k = (j, i)
+      const double2 ha = herm(q.ma, kx, kyj, ik0[u], kxm, kyjm, ik1[u], A0[u], A1[u]);
+      const double2 hb = herm(q.mb, kx, kyj, ik0[u], kxm, kyjm, ik1[u], A0[u], A1[u]);
+      S[i] = make_double2(ha.x - hb.y, ha.y + hb.x);  // ha + i hb
+      // S at -k = (jm, im); for a self-paired row this rewrites the same value
+      const double2 hc = herm(q.ma, kxm, kyjm, ik1[u], kx, kyj, ik0[u], A1[u], A0[u]);
+      const double2 hd = herm(q.mb, kxm, kyjm, ik1[u], kx, kyj, ik0[u], A1[u], A0[u]);
+      S[(nrow - 1) * nx + im] = make_double2(hc.x - hd.y, hc.y + hd.x);
     }
     __syncthreads();
-    lds_ifft<kRowThreads, 8>(S, nx, a.logx, nrow, nx, 1, a.twx);
-    double2* w = a.work + ((size_t)(ti * a.nq + qi) * a.ny) * nx;
+    lds_ifft<kRowThreads, 4>(S, nx, a.logx, nrow, nx, 1, a.twx);
+    double2* w = a.work + (size_t)(ti * a.nq + qi) * a.ny * nx;
     for (int e = tid; e < nrow * nx; e += kRowThreads) {
       const int r = e / nx, i = e - r * nx;
       const int row = r == 0 ? j : jm;
-      w[(size_t)row * nx + i] = S[r * nx + i];
+      const int xb = i / cb, c = i - xb * cb;
+      w[((size_t)xb * a.ny + row) * cb + c] = S[e];
     }
     __syncthreads();
   }
 }
 
+// Pass 2: one workgroup per (packed spectrum, block of cb columns, snapshot):
+// the block is one contiguous run of the workspace; FFT along y in LDS, then
+// scaled by 1/(Nx Ny) and written as float32 fields, x-major (field[x][y]).
 __global__ __launch_bounds__(kColThreads) void synth_cols_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double2* C = (double2*)lds_raw;  // [ny][cb]
   const int cb = a.cb;
-  const int x0 = blockIdx.x * cb;
+  const int xb = blockIdx.x;
+  const int x0 = xb * cb;
   const int qi = blockIdx.y;
   const int ti = blockIdx.z;
   const int tid = threadIdx.x;
   const int ny = a.ny, nx = a.nx;
-  const double2* w = a.work + ((size_t)(ti * a.nq + qi) * ny) * nx;
-  for (int e = tid; e < ny * cb; e += kColThreads) {
-    const int y = e / cb, c = e - y * cb;
-    C[e] = w[(size_t)y * nx + x0 + c];
-  }
+  const double2* w = a.work + (size_t)(ti * a.nq + qi) * ny * nx + (size_t)xb * ny * cb;
+  for (int e = tid; e < ny * cb; e += kColThreads) C[e] = w[e];
   __syncthreads();
   lds_ifft<kColThreads, 8>(C, ny, a.logy, cb, 1, cb, a.twy);
   const QPlan q = a.q[qi];
@@ -254,7 +285,6 @@ __global__ __launch_bounds__(kColThreads) void synth_cols_kernel(SynthArgs a) {
     }
   }
 }
-
 int ilog2(int n) {
   int l = 0;
   while ((1 << l) < n) ++l;
@@ -380,7 +410,7 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   if (cb > d->nx) cb = d->nx;
   a.cb = cb;
   hipStream_t s = (hipStream_t)stream;
-  const size_t lds_rows = 4 * (size_t)d->nx * sizeof(double2);
+  const size_t lds_rows = 2 * (size_t)d->nx * sizeof(double2);
   const size_t lds_cols = (size_t)cb * d->ny * sizeof(double2);
   OSAR_HIP(hipFuncSetAttribute((const void*)synth_rows_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows));
