@@ -9,11 +9,12 @@ resident in HBM before the timed region; the raw cube stays on the device.
 Metric: facet-pulse contributions per second (one facet's echo deposited into
 one pulse's range line for one channel and polarisation).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 3]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large]
 """
 import argparse
 import json
 import os
+import re
 import sys
 import tempfile
 import time
@@ -36,6 +37,32 @@ def flop_per_contribution(n_ch, n_pol):
     return 38. + 35. / n_ch
 
 
+def configs_names():
+    from oceansar_amd.configs import WORKLOADS
+    return WORKLOADS
+
+
+def measured_traffic(sc, shape_repr, pulses_per_launch):
+    """HBM bytes per echo launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (counters cannot be read from inside the timed run).
+    Returns (bytes or None, source file); None when the summary was taken at a
+    different launch shape or launch size."""
+    prof = os.path.join(ROOT, 'profiles')
+    try:
+        names = [n for n in os.listdir(prof)
+                 if n == 'raw_generation_%dx%d_%dpulses_%dch_%s_hbm_traffic.md'
+                 % (sc.Nx, sc.Ny, sc.az_steps, sc.num_ch, sc.pol)]
+    except OSError:
+        return None, None
+    for n in names:
+        text = open(os.path.join(prof, n)).read()
+        m = re.search(r'traffic_bytes_per_launch echo_accumulate_kernel = ([0-9.eE+-]+)', text)
+        s = re.search(r'pulses_per_launch = (\d+), launch_shape = (.*)', text)
+        if m and s and int(s.group(1)) == pulses_per_launch and s.group(2).strip() == shape_repr:
+            return float(m.group(1)), 'profiles/' + n
+    return None, None
+
+
 def cpu_baseline(workload, sample_pulses, threads):
     """The C restatement of the reference loop (plus the numpy FP64 synthesis and
     Bragg RCS it needs) timed on host cores over a bounded pulse sample."""
@@ -43,7 +70,7 @@ def cpu_baseline(workload, sample_pulses, threads):
     from oceansar_amd import configs
     import oracle.c_oracle as co
     import oracle.sar as osar
-    sc = configs.load_scene(configs.baseline(workload), tempfile.mkdtemp())
+    sc = configs.load_scene(configs.workload(workload), tempfile.mkdtemp())
     p0 = sc.az_steps // 2
     p1 = p0 + sample_pulses
     t = time.perf_counter()
@@ -51,7 +78,7 @@ def cpu_baseline(workload, sample_pulses, threads):
     dt = time.perf_counter() - t
     w = sc.Nx * sc.Ny * (p1 - p0) * sc.num_ch * len(sc.pols)
     return {'value': w / dt, 'unit': 'contributions/s', 'cores': threads, 'kind': 'port',
-            'sample': 'workload %d, pulses [%d, %d) of %d (%d snapshots, %d Bragg refreshes), '
+            'sample': 'workload %s, pulses [%d, %d) of %d (%d snapshots, %d Bragg refreshes), '
                       '%.1f s' % (workload, p0, p1, sc.az_steps, (p1 - 1) // sc.K - p0 // sc.K + 2,
                                   len(set(sc.rcs_slot[p0:p1])), dt)}
 
@@ -61,11 +88,14 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--workload', type=int, default=3)
+    ap.add_argument('--workload', default='large', choices=configs_names())
     ap.add_argument('--pulses', type=int, default=None, help='shorter scene (testing only)')
     ap.add_argument('--chunk', type=int, default=32, help='snapshot intervals per launch group')
     ap.add_argument('--cpu-sample', type=int, default=32)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-probe-mib', type=int, default=0,
+                    help='after timing, run one device copy of this many MiB (a known-size '
+                         'kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path '
@@ -77,7 +107,7 @@ def main():
     from oceansar_amd.engine import DeviceScene
 
     over = {} if args.pulses is None else {'n_pulses': args.pulses}
-    sc = configs.load_scene(configs.baseline(args.workload, **over), tempfile.mkdtemp())
+    sc = configs.load_scene(configs.workload(args.workload, **over), tempfile.mkdtemp())
     ds = DeviceScene(sc)
     W = sc.contributions
     raw = ds.new_raw()
@@ -111,6 +141,8 @@ def main():
     n_launch = len(echo_ms)
     contrib_per_launch = W / (n_launch / args.steps)
     achieved = contrib_per_launch * fpc / (np.mean(echo_ms) * 1e-3) / 1e12
+    pulses_per_launch = int(round(sc.az_steps / (n_launch / args.steps)))
+    traffic, traffic_src = measured_traffic(sc, repr(ds.shape), pulses_per_launch)
 
     out = {
         'metric': 'facet-pulse contributions/s (raw-data generation, rel. RMS <= 1e-4 vs FP64)',
@@ -125,7 +157,7 @@ def main():
         'vs_baseline': None,
         'dtype': 'f32 (FP64 range/phase/bins)',
         'data': 'synthetic (seeded Elfouhaily surface, seeded scatterers)',
-        'config': {'workload': 'workload %d: %dx%d facets, %d pulses, %d channels, %s, K=%d'
+        'config': {'workload': '%s: %dx%d facets, %d pulses, %d channels, %s, K=%d'
                                % (args.workload, sc.Nx, sc.Ny, sc.az_steps, sc.num_ch, sc.pol,
                                   sc.K),
                    'facets': sc.Nx * sc.Ny, 'pulses': sc.az_steps, 'channels': sc.num_ch,
@@ -134,12 +166,24 @@ def main():
                    'echo_launch_shape': repr(ds.shape)},
         'roofline': {'bound': 'mfma', 'kernel': 'echo_accumulate_kernel',
                      'achieved': achieved, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': achieved / FP32_PEAK_TFLOPS, 'traffic': None,
+                     'frac': achieved / FP32_PEAK_TFLOPS, 'traffic': traffic,
+                     'traffic_source': traffic_src,
+                     'traffic_gbs': (None if traffic is None
+                                     else traffic / (np.mean(echo_ms) * 1e-3) / 1e9),
                      'flop_per_contribution': fpc,
+                     'pulses_per_launch': pulses_per_launch,
                      'avg_launch_ms': float(np.mean(echo_ms)),
                      'note': 'FP32 vector-ALU bound (VALU + transcendental + LDS atomics); '
                              'peak is the MI355X FP32 figure'},
     }
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.neg(src, out=dst)      # an elementwise kernel: reads and writes n*4 bytes
+        torch.cuda.synchronize()
+        del src, dst
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         out['cpu_baseline'] = cpu_baseline(args.workload, args.cpu_sample, threads)

@@ -133,14 +133,18 @@ def cfg_text(L, dx, n_pulses, K, U, wind_dir=0., num_ch=1, b_ati=None, pol='vv',
         spread=spread, choppy=choppy, specular=specular)
 
 
-def baseline(index, **over):
-    """cfg text for the workloads this build is measured on (index 1..3).
+WORKLOADS = ('small', 'ati_swell', 'large')
 
-    1: 256^2 facets at 1 m, 1 channel, 1024 pulses, K=16, U10=10, cos^2 spread.
-    2: 1024^2 at 0.5 m, 2-channel ATI (0/10 m), 4096 pulses, K=8, U10=7 at 45 deg,
-       200 m directional swell (Hs 2 m), Bragg + specular.
-    3: 2048^2 at 0.5 m, 4 channels (0/5/10/20 m), 8192 pulses, K=16, U10=12.
+
+def workload(name, **over):
+    """cfg text for the measured workloads; keyword overrides go to cfg_text.
+
+    small:     256^2 facets at 1 m, 1 channel, 1024 pulses, K=16, U10=10, cos^2 spread.
+    ati_swell: 1024^2 at 0.5 m, 2-channel ATI (0/10 m), 4096 pulses, K=8, U10=7 at
+               45 deg, 200 m directional swell (Hs 2 m), Bragg + specular.
+    large:     2048^2 at 0.5 m, 4 channels (0/5/10/20 m), 8192 pulses, K=16, U10=12.
     """
+    index = {'small': 1, 'ati_swell': 2, 'large': 3}.get(name)
     if index == 1:
         kw = dict(L=256., dx=1., n_pulses=1024, K=16, U=10., wind_dir=0., num_ch=1,
                   spread='cos2s', seed=1)
@@ -152,7 +156,7 @@ def baseline(index, **over):
         kw = dict(L=1024., dx=0.5, n_pulses=8192, K=16, U=12., wind_dir=0., num_ch=4,
                   b_ati=[0., 5., 10., 20.], seed=3)
     else:
-        raise ValueError('unknown workload %r' % index)
+        raise ValueError('unknown workload %r (known: %s)' % (name, ', '.join(WORKLOADS)))
     kw.update(over)
     return cfg_text(**kw)
 

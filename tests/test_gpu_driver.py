@@ -12,7 +12,7 @@ def test_sar_raw_file_matches_oracle(need_gpu, tmp_path):
     from oceansar_amd.scene import SarScene
     from oceansar_amd.utils import db2lin
     import oracle.sar as osar
-    text = configs.baseline(1, L=64., n_pulses=96, K=8, num_ch=2, b_ati=[0., 10.])
+    text = configs.workload('small', L=64., n_pulses=96, K=8, num_ch=2, b_ati=[0., 10.])
     cfg_path = configs.write(str(tmp_path / 'sim.cfg'), text)
     out = str(tmp_path / 'raw.nc')
     ocean = str(tmp_path / 'ocean.nc')

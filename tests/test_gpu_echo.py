@@ -26,7 +26,7 @@ def phase_rms(a, b, floor_db=-30.):
 @pytest.fixture(scope='module')
 def cfg1_scene(tmp_path_factory):
     from oceansar_amd import configs
-    return configs.load_scene(configs.baseline(1), tmp_path_factory.mktemp('cfg1'))
+    return configs.load_scene(configs.workload('small'), tmp_path_factory.mktemp('cfg1'))
 
 
 def _snapshots(sc, m0, m1):

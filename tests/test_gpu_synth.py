@@ -28,8 +28,8 @@ def phase_rms(a, b, floor_db=-30.):
 
 def _scene(tmp_path_factory, name, **kw):
     from oceansar_amd import configs
-    base = kw.pop('base', 1)
-    return configs.load_scene(configs.baseline(base, **kw), tmp_path_factory.mktemp(name))
+    base = kw.pop('base', 'small')
+    return configs.load_scene(configs.workload(base, **kw), tmp_path_factory.mktemp(name))
 
 
 def _synth_compare(sc, times, names, compute):
@@ -64,7 +64,7 @@ def test_synth_matches_oracle_square(need_gpu, tmp_path_factory):
 def test_synth_matches_oracle_rectangular_choppy(need_gpu, tmp_path_factory):
     """Nx != Ny, current, finite depth and choppy slopes."""
     from oceansar_amd import configs
-    text = configs.baseline(1, L=128., dx=1., choppy=True)
+    text = configs.workload('small', L=128., dx=1., choppy=True)
     text = text.replace('Ly=128.0', 'Ly=64.0').replace('current_mag=0', 'current_mag=0.4')
     text = text.replace('depth=1000', 'depth=20')
     sc = configs.load_scene(text, tmp_path_factory.mktemp('s2'))
@@ -112,12 +112,12 @@ def test_end_to_end_dp_multichannel_xti(need_gpu, tmp_path_factory):
 
 @pytest.mark.parametrize('pol', ['vv', 'DP'])
 def test_end_to_end_specular(need_gpu, tmp_path_factory, pol):
-    """Bragg + Kirchhoff stationary-phase specular (workload-2 physics at 8 deg incidence,
+    """Bragg + Kirchhoff stationary-phase specular (ati_swell workload physics at 8 deg incidence,
     where stationary-phase facets exist), two ATI channels."""
     import torch
     import oracle.sar as osar
     from oceansar_amd.engine import DeviceScene
-    sc = _scene(tmp_path_factory, 'k' + pol, base=2, L=128., n_pulses=512, inc=8., pol=pol)
+    sc = _scene(tmp_path_factory, 'k' + pol, base='ati_swell', L=128., n_pulses=512, inc=8., pol=pol)
     assert sc.spec_enable
     ds = DeviceScene(sc)
     p0, p1 = 250, 262

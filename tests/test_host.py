@@ -43,12 +43,12 @@ Lx=512
 
 
 @pytest.mark.parametrize('idx,nx,pulses,orig,samp,nscat', [
-    (1, 256, 1024, 115, 128, 38400), (2, 1024, 4096, 267, 512, 54000),
-    (3, 2048, 8192, 624, 1024, 54000)])
+    ('small', 256, 1024, 115, 128, 38400), ('ati_swell', 1024, 4096, 267, 512, 54000),
+    ('large', 2048, 8192, 624, 1024, 54000)])
 def test_workload_sizes(tmp_path, idx, nx, pulses, orig, samp, nscat):
     """Sizes derived by the reference formulas (sar_raw_nompi.py:264-278)."""
     from oceansar_amd import configs
-    sc = configs.load_scene(configs.baseline(idx), tmp_path)
+    sc = configs.load_scene(configs.workload(idx), tmp_path)
     assert (sc.Nx, sc.Ny, sc.az_steps) == (nx, nx, pulses)
     # the pulse count also follows from t_span without the n_pulses override
     assert int(np.floor(sc.t_span / sc.t_step)) == pulses
@@ -61,7 +61,7 @@ def test_workload_sizes(tmp_path, idx, nx, pulses, orig, samp, nscat):
 def test_rcs_schedule_float_cadence(tmp_path):
     """Without rcs_update_every the refresh follows t_now - t_last > dt (sar_raw_nompi.py:432)."""
     from oceansar_amd import configs
-    text = configs.baseline(1, K=1, dt=0.05)
+    text = configs.workload('small', K=1, dt=0.05)
     sc = configs.load_scene(text, tmp_path)
     ref, t_last = [], -1.
     for p in range(sc.az_steps):
@@ -75,9 +75,9 @@ def test_rcs_schedule_float_cadence(tmp_path):
 
 def test_seeded_inputs_reproducible(tmp_path):
     from oceansar_amd import configs
-    a = configs.load_scene(configs.baseline(1), tmp_path)
-    b = configs.load_scene(configs.baseline(1), tmp_path)
-    c = configs.load_scene(configs.baseline(1, seed=9), tmp_path)
+    a = configs.load_scene(configs.workload('small'), tmp_path)
+    b = configs.load_scene(configs.workload('small'), tmp_path)
+    c = configs.load_scene(configs.workload('small', seed=9), tmp_path)
     np.testing.assert_array_equal(a.surface.wave_coefs, b.surface.wave_coefs)
     np.testing.assert_array_equal(a.scat_m.ind0, b.scat_m.ind0)
     assert not np.array_equal(a.surface.wave_coefs, c.surface.wave_coefs)
@@ -87,7 +87,7 @@ def test_surface_statistics(tmp_path):
     """E[var Dz] = sum of the directional spectrum; slopes' variance ~ rmss^2 (ocean.py:181)."""
     from oceansar_amd import configs
     import oracle.surface as osurf
-    sc = configs.load_scene(configs.baseline(1), tmp_path)
+    sc = configs.load_scene(configs.workload('small'), tmp_path)
     s = sc.surface
     f = osurf.synth(s, 0.0, ('D', 'Diff'))
     dk2 = float(s.kx_res * s.ky_res)
@@ -129,7 +129,7 @@ def test_c_oracle_matches_numpy_oracle(tmp_path):
     from oceansar_amd import configs
     import oracle.c_oracle as co
     import oracle.sar as osar
-    sc = configs.load_scene(configs.baseline(1, L=96., pol='DP', num_ch=2, b_ati=[0, 10],
+    sc = configs.load_scene(configs.workload('small', L=96., pol='DP', num_ch=2, b_ati=[0, 10],
                                              b_xti=[0, 25.], K=4), tmp_path)
     cache = osar.SnapshotCache(sc)
     a, na = osar.simulate(sc, 300, 330, snaps=cache)

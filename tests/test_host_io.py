@@ -32,7 +32,7 @@ def test_rawfile_layout_and_round_trip(tmp_path):
 def test_ocean_state_round_trip(tmp_path):
     from oceansar_amd import configs
     from oceansar_amd.ocean import OceanSurface
-    sc = configs.load_scene(configs.baseline(1, L=64.), tmp_path)
+    sc = configs.load_scene(configs.workload('small', L=64.), tmp_path)
     s = OceanSurface()
     s._adopt(sc.surface, ['D'])
     p = str(tmp_path / 'ocean.nc')
