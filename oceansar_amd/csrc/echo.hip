@@ -36,6 +36,10 @@ constexpr int kThreads = 512;
 constexpr int kTaps = 8;
 constexpr int kOvs = 20;
 constexpr int kTab = kTaps * kOvs + 1;  // 161
+// LDS copy of the table, sub-bin major: row sb holds the 8 taps
+// sinc_tab[sb + 20 j], j = 0..7, so a facet reads its taps with two 16-byte loads.
+constexpr int kTabRows = kOvs + 1;       // 21
+constexpr int kTabLds = kTabRows * kTaps;  // 168 floats
 
 __device__ __forceinline__ double np_sinc(double x) {
   // numpy.sinc: sin(pi x)/(pi x) with x = 0 replaced by 1e-20
@@ -181,19 +185,43 @@ __device__ bool ka_spa(double k0, int pol_code, double th, double ph, double Dx,
   return true;
 }
 
-// Per-lane inputs of one facet (one column x, the lane's row and pulse).
+// Scatterer series offsets of one facet. They are loaded one column ahead of
+// the facet's other inputs, so that the gather of the scatterer values (which
+// depends on them) never waits for a load issued in the same column.
+struct FacetIdx {
+  int m, p;
+};
+
+__device__ __forceinline__ void load_index(const osar_echo_desc& d, size_t idx, bool bragg,
+                                           FacetIdx& fi) {
+  if (bragg) {
+    fi.m = d.ind0_m[idx];
+    fi.p = d.ind0_p[idx];
+  }
+}
+
+// Per-lane inputs of one facet (one column x, the lane's row and pulse),
+// including the column's geometry (the same for every lane): all of it is
+// loaded one column ahead, so no load is waited for in the column that issues it.
 template <int NPOL, bool SPEC>
 struct FacetIn {
+  double sr, ci, si, cap, inc, dinc;  // sr_col, cos_inc, sin_inc, cap_rate, inc_col, dinc_col
   float dx0, dy0, dz0, dx1, dy1, dz1;
   float amp[NPOL][2];
   osar_c32 sm, sp;
   float s0[5], s1[5];  // specular slope / curvature fields at m and m+1
 };
 
-template <int NPOL, bool SPEC>
-__device__ __forceinline__ void load_facet(const osar_echo_desc& d, size_t idx, size_t so,
+template <int NPOL, bool SPEC, bool XTI>
+__device__ __forceinline__ void load_facet(const osar_echo_desc& d, int x, size_t idx, size_t so,
                                            size_t plane, int rk, int p, int rslot, bool bragg,
-                                           FacetIn<NPOL, SPEC>& f) {
+                                           const FacetIdx& fi, FacetIn<NPOL, SPEC>& f) {
+  f.sr = d.sr_col[x];
+  f.ci = d.cos_inc[x];
+  f.si = d.sin_inc[x];
+  if (bragg) f.cap = d.cap_rate[x];
+  if (SPEC) f.inc = d.inc_col[x];
+  if (XTI) f.dinc = d.dinc_col[x];
   f.dx0 = d.dx[so];
   f.dy0 = d.dy[so];
   f.dz0 = d.dz[so];
@@ -209,8 +237,8 @@ __device__ __forceinline__ void load_facet(const osar_echo_desc& d, size_t idx, 
       f.amp[pol][0] = d.amp[ao];
       f.amp[pol][1] = d.amp[ao + plane];
     }
-    f.sm = d.scat_m[d.ind0_m[idx] + p];
-    f.sp = d.scat_p[d.ind0_p[idx] + p];
+    f.sm = d.scat_m[fi.m + p];
+    f.sp = d.scat_p[fi.p + p];
   }
   if (SPEC) {
     const float* src[5] = {d.sx, d.sy, d.sxx, d.syy, d.sxy};
@@ -257,42 +285,77 @@ __device__ __forceinline__ void deposit(const Lines& L, int pol, int ch, int nch
 }
 
 // Flush one channel-independent window slot: multiply by each channel's
-// per-(row, pulse) factor and deposit.
+// per-(row, pulse) factor and deposit. The window test is made once per slot;
+// inside the window every (pol, ch) line takes its two LDS adds without further
+// branches (lines of channels >= nch exist in LDS and receive zeros, as their
+// factors are zero, and are never written back).
 template <int P, int NCH, int NPOL>
 __device__ __forceinline__ void flush_slot(const Lines& L, const float2 (&s)[NPOL],
                                            const float2 (&fch)[NCH], int nch, int pi, int p,
                                            int b) {
+  if (b >= L.wb && b < L.wend) {
+    double* base = L.acc + (size_t)pi * L.LS + (b - L.wb);
 #pragma unroll
-  for (int pol = 0; pol < NPOL; ++pol) {
-    if (s[pol].x == 0.f && s[pol].y == 0.f) continue;
+    for (int pol = 0; pol < NPOL; ++pol) {
 #pragma unroll
-    for (int ch = 0; ch < NCH; ++ch) {
-      if (ch >= nch) break;
-      deposit<P, NCH>(L, pol, ch, nch, pi, p, b, osar::cmul(s[pol], fch[ch]));
+      for (int ch = 0; ch < NCH; ++ch) {
+        const float2 v = osar::cmul(s[pol], fch[ch]);
+        double* l = base + (size_t)((pol * NCH + ch) * P) * L.LS;
+        atomicAdd(l, (double)v.x);
+        atomicAdd(l + L.W, (double)v.y);
+      }
+    }
+  } else if (b >= 0 && b < L.nbins) {
+#pragma unroll
+    for (int pol = 0; pol < NPOL; ++pol) {
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        if (ch >= nch) break;
+        const float2 v = osar::cmul(s[pol], fch[ch]);
+        osar_c32* o = L.raw + ((size_t)(pol * nch + ch) * L.npt + p) * L.nbins + b;
+        unsafeAtomicAdd(&o->re, v.x);
+        unsafeAtomicAdd(&o->im, v.y);
+      }
     }
   }
 }
 
 // XTI: the cross-track phase varies per facet, so channels cannot share a
 // window; this template variant deposits every tap of every channel.
+// Workgroups are dispatched to the 8 XCDs round-robin, and each XCD has its
+// own L2. Work item w = (pulse group fastest, then row tile) is remapped so
+// that each XCD runs a contiguous range of items: the pulse groups of one
+// snapshot interval, which read the same surface rows, then share an L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_item(int h, int G) {
+  const int q = G / kXcds, r = G % kXcds;
+  const int j = h % kXcds, s = h / kXcds;
+  return j * q + min(j, r) + s;
+}
+
 template <int P, int NCH, int NPOL, bool SPEC, bool XTI>
 __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_desc d, int W,
-                                                                   int R, int LS) {
+                                                                   int R, int LS, int n_pgroups,
+                                                                   int n_items) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int NL = NPOL * NCH * P;
   double* acc = (double*)lds_raw;                  // [NL][LS]
-  float* tab = (float*)(acc + (size_t)NL * LS);    // [kTab]
-  int* wlo = (int*)(tab + 164);                    // [P]
+  float* tab = (float*)(acc + (size_t)NL * LS);    // [kTabRows][kTaps]
+  int* wlo = (int*)(tab + kTabLds);                // [P]
 
   const int tid = threadIdx.x;
-  const int pstart = d.pulse0 + blockIdx.x * P;
+  const int item = xcd_item(blockIdx.x, n_items);
+  const int pstart = d.pulse0 + (item % n_pgroups) * P;
   const int pend = d.pulse0 + d.npulses;
-  const int y0 = blockIdx.y * R;
+  const int y0 = (item / n_pgroups) * R;
   const int rows = min(R, d.ny - y0);
   const int nch = d.n_ch;
 
   for (int i = tid; i < NL * LS; i += kThreads) acc[i] = 0.;
-  for (int i = tid; i < kTab; i += kThreads) tab[i] = d.sinc_tab[i];
+  for (int i = tid; i < kTabLds; i += kThreads) {
+    const int sb = i / kTaps, j = i - sb * kTaps;
+    tab[i] = d.sinc_tab[sb + kOvs * j];
+  }
   if (tid < P) {
     // LDS window start for this pulse: smallest migration term over the tile's rows
     // (monotonic in |az| on either side of az = 0)
@@ -337,6 +400,7 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
   L.npt = d.n_pulses_total;
   L.raw = d.raw;
   const double two_k0 = 2 * d.k0;
+  const double inv_binsize = 1.0 / d.binsize;
   const int spol = NPOL == 2 ? 2 : (d.pol == 1 ? 1 : 0);
   double spec_acc = 0.;
 
@@ -366,19 +430,28 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
       for (int j = 0; j < kTaps; ++j) win[pol][j] = make_float2(0.f, 0.f);
     int wbase = INT_MIN;
 
+    // software pipeline: inputs of column x + 1 and offsets of column x + 2
+    // are in flight while column x is processed
     FacetIn<NPOL, SPEC> cur, nxt;
-    load_facet<NPOL, SPEC>(d, (size_t)yrow, snap_off + yrow, plane, rk, p, rslot, bragg_ok, cur);
+    FacetIdx ia = {0, 0}, ib = {0, 0};
+    load_index(d, (size_t)yrow, bragg_ok, ia);
+    load_facet<NPOL, SPEC, XTI>(d, 0, (size_t)yrow, snap_off + yrow, plane, rk, p, rslot,
+                                bragg_ok, ia, cur);
+    if (d.nx > 1) load_index(d, (size_t)d.ny + yrow, bragg_ok, ia);
     for (int x = 0; x < d.nx; ++x) {
       const size_t idx = (size_t)x * d.ny + yrow;
-      if (x + 1 < d.nx)   // prefetch the next column while this one is processed
-        load_facet<NPOL, SPEC>(d, idx + d.ny, snap_off + idx + d.ny, plane, rk, p, rslot,
-                               bragg_ok, nxt);
+      if (x + 1 < d.nx) {
+        ib = ia;
+        if (x + 2 < d.nx) load_index(d, idx + 2 * (size_t)d.ny, bragg_ok, ia);
+        load_facet<NPOL, SPEC, XTI>(d, x + 1, idx + d.ny, snap_off + idx + d.ny, plane, rk, p,
+                                    rslot, bragg_ok, ib, nxt);
+      }
       const double Dx = lerp2(cur.dx0, cur.dx1, rk, w);
       const double Dy = lerp2(cur.dy0, cur.dy1, rk, w);
       const double Dz = lerp2(cur.dz0, cur.dz1, rk, w);
-      const Geo g = facet_range(d.sr_col[x], d.cos_inc[x], d.sin_inc[x], mig, sin_az, Dx, Dy,
+      const Geo g = facet_range(cur.sr, cur.ci, cur.si, mig, sin_az, Dx, Dy,
                                 Dz, d.sr0);
-      const double xb = (g.srs - d.min_sr) / d.binsize;
+      const double xb = osar::div_by(g.srs - d.min_sr, d.binsize, inv_binsize);
       const bool finite = fabs(xb) < 1.0e9;   // non-finite or absurd range: no valid tap
       const double fl = finite ? floor(xb) : 0.;
       const int dbin = (int)fl;
@@ -389,7 +462,7 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
       for (int pol = 0; pol < NPOL; ++pol) scene[pol] = make_float2(0.f, 0.f);
       if (bragg_ok) {
         const double surf = -two_k0 * g.srs;
-        const double cap = d.cap_rate[x] * (double)(p + 1);
+        const double cap = cur.cap * (double)(p + 1);
         const float2 e0 = osar::cis_fp64(surf - cap);
         const float2 e1 = osar::cis_fp64(surf + cap);
         const float2 b0 = osar::cmul(e0, make_float2(cur.sm.re, cur.sm.im));
@@ -403,7 +476,7 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
       if (SPEC) {
         // Esn = exp(-i 2 k0 sr_surface) (4 pi)^1.5 E   (sar_raw_nompi.py:405-426)
         zd e[2];
-        if (finite && ka_spa(d.k0, spol, d.inc_col[x], phi, lerp2(cur.s0[0], cur.s1[0], rk, w),
+        if (finite && ka_spa(d.k0, spol, cur.inc, phi, lerp2(cur.s0[0], cur.s1[0], rk, w),
                              lerp2(cur.s0[1], cur.s1[1], rk, w),
                              lerp2(cur.s0[2], cur.s1[2], rk, w),
                              lerp2(cur.s0[3], cur.s1[3], rk, w),
@@ -456,13 +529,15 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
             }
             wbase = want;
           }
+          const float4 t0 = *(const float4*)(tab + sb * kTaps);
+          const float4 t1 = *(const float4*)(tab + sb * kTaps + 4);
+          const float t[kTaps] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
 #pragma unroll
           for (int j = 0; j < kTaps; ++j) {
-            const float t = tab[sb + kOvs * j];
 #pragma unroll
             for (int pol = 0; pol < NPOL; ++pol) {
-              win[pol][j].x = fmaf(t, scene[pol].x, win[pol][j].x);
-              win[pol][j].y = fmaf(t, scene[pol].y, win[pol][j].y);
+              win[pol][j].x = fmaf(t[j], scene[pol].x, win[pol][j].x);
+              win[pol][j].y = fmaf(t[j], scene[pol].y, win[pol][j].y);
             }
           }
         } else {
@@ -471,14 +546,14 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
             if (ch >= nch) break;
             float2 f = fch[ch];
             if (d.b_xti[ch] != 0.0) {
-              const double tot = d.dinc_col[x] + g.dinc;
+              const double tot = cur.dinc + g.dinc;
               f = osar::cmul(f, osar::cis_fp64(-(d.k0 * d.b_xti[ch]) * tot));
             }
 #pragma unroll
             for (int pol = 0; pol < NPOL; ++pol) {
               const float2 v = osar::cmul(scene[pol], f);
               for (int j = 0; j < kTaps; ++j) {
-                const float t = tab[sb + kOvs * j];
+                const float t = tab[sb * kTaps + j];
                 deposit<P, NCH>(L, pol, ch, nch, pi, p, dbin - 4 + j,
                                 make_float2(t * v.x, t * v.y));
               }
@@ -525,7 +600,8 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
 }
 
 size_t echo_lds_bytes(int P, int nch_c, int npol, int LS) {
-  return sizeof(double) * (size_t)npol * nch_c * P * LS + sizeof(float) * 164 + sizeof(int) * P;
+  return sizeof(double) * (size_t)npol * nch_c * P * LS + sizeof(float) * kTabLds +
+         sizeof(int) * P;
 }
 
 int line_stride(int W) {
@@ -543,9 +619,11 @@ int launch_accumulate(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipS
                "pulses_per_block", lds, ctx->max_lds);
   OSAR_HIP(hipFuncSetAttribute((const void*)echo_accumulate_kernel<P, NCH, NPOL, SPEC, XTI>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  dim3 grid((d->npulses + P - 1) / P, (d->ny + R - 1) / R);
-  hipLaunchKernelGGL((echo_accumulate_kernel<P, NCH, NPOL, SPEC, XTI>), grid, dim3(kThreads),
-                     lds, s, *d, W, R, LS);
+  const int n_pgroups = (d->npulses + P - 1) / P;
+  const int64_t n_items = (int64_t)n_pgroups * ((d->ny + R - 1) / R);
+  OSAR_REQUIRE(n_items < INT_MAX, OSAR_EINVAL, "too many workgroups (%lld)", (long long)n_items);
+  hipLaunchKernelGGL((echo_accumulate_kernel<P, NCH, NPOL, SPEC, XTI>), dim3((unsigned)n_items),
+                     dim3(kThreads), lds, s, *d, W, R, LS, n_pgroups, (int)n_items);
   OSAR_LAUNCHED("echo_accumulate_kernel");
   return OSAR_OK;
 }
