@@ -157,7 +157,14 @@ typedef struct osar_bragg_desc {
   double k0, d, wind_dir, wind_U, wind_fetch;
   double dx, dy;            /* facet size (RCS = NRCS * dx * dy) */
   double t_step, t_span, v_ground, gr0;
-  const double* inc_col;    /* [nx] incidence angle */
+  int32_t spec_uniform;     /* 1: spec_k[i] == spec_k0 + i * spec_dk exactly (product
+                               rounded first, as numpy.arange fills), so the search
+                               computes grid points instead of loading them */
+  int32_t reserved;
+  double spec_k0, spec_dk;  /* spec_k[0] and the grid step (spec_uniform) */
+  double spec_inv_dk;       /* 1 / mean step of spec_k: starting guess of the search */
+  const double* cos_inc;    /* [nx] cos and sin of the incidence angle */
+  const double* sin_inc;
   const double* y_row;      /* [ny] */
   const double* spec_k;     /* [n_spec] ascending */
   const double* spec_S;     /* [n_spec] */

@@ -45,8 +45,10 @@ class BraggDesc(ctypes.Structure):
     _fields_ = [(n, c_int32) for n in ('nx', 'ny', 'pol', 'spread', 'n_spec', 'pulse')] + \
         [(n, c_double) for n in ('k0', 'd', 'wind_dir', 'wind_U', 'wind_fetch', 'dx', 'dy',
                                  't_step', 't_span', 'v_ground', 'gr0')] + \
-        [(n, c_void_p) for n in ('inc_col', 'y_row', 'spec_k', 'spec_S', 'diffx', 'diffy',
-                                 'amp', 'rcs_sum')]
+        [(n, c_int32) for n in ('spec_uniform', 'reserved')] + \
+        [(n, c_double) for n in ('spec_k0', 'spec_dk', 'spec_inv_dk')] + \
+        [(n, c_void_p) for n in ('cos_inc', 'sin_inc', 'y_row', 'spec_k', 'spec_S', 'diffx',
+                                 'diffy', 'amp', 'rcs_sum')]
 
 
 class SurfaceDesc(ctypes.Structure):

@@ -7,6 +7,8 @@
 // (oceansar/radarsim/sar_raw_nompi.py:432-467). One thread per facet,
 // coalesced struct-of-arrays reads of the two slope fields; the spectrum
 // table lookup reproduces scipy interp1d(kind='linear', fill_value=1).
+#include <cmath>
+
 #include "common.h"
 
 namespace {
@@ -19,152 +21,225 @@ __device__ __forceinline__ cd cmulc(cd a, cd b) {
   return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 __device__ __forceinline__ cd cscale(cd a, double s) { return {a.re * s, a.im * s}; }
-__device__ __forceinline__ cd cdiv(cd a, cd b) {
-  // Smith's algorithm, as numpy divides complex128
-  if (fabs(b.re) >= fabs(b.im)) {
-    const double rat = b.im / b.re;
-    const double scl = 1.0 / (b.re + b.im * rat);
-    return {(a.re + a.im * rat) * scl, (a.im - a.re * rat) * scl};
-  }
-  const double rat = b.re / b.im;
-  const double scl = 1.0 / (b.im + b.re * rat);
-  return {(a.re * rat + a.im) * scl, (a.im * rat - a.re) * scl};
-}
-__device__ __forceinline__ double cabs2(cd a) {
-  const double h = hypot(a.re, a.im);
-  return h * h;
-}
+__device__ __forceinline__ double cabs2(cd a) { return a.re * a.re + a.im * a.im; }
 
 constexpr double kG = 9.80665;
 constexpr double kRho = 1000.;
 constexpr double kSurfT = 0.072;
 constexpr double kX0 = 22e3;
 
-// Delta(k) of the Elfouhaily spreading (oceansar/spread/elfouhaily.py:26-52).
-__device__ double elf_delta(double k, double U, double fetch) {
+// Facet-independent terms of the Elfouhaily Delta(k) (oceansar/spread/
+// elfouhaily.py:26-52) and of the geometry, computed once per call on the host.
+struct BraggConst {
+  double cK_p, cK_m, a_0, a_m;   // Delta(k) = tanh(a_0 + 4 (cK/cK_p)^2.5 + a_m (cK_m/cK)^2.5)
+  double cos_wd, sin_wd;         // wind direction
+  double sq_re, sq_im;           // sqrt(epsilon_sw), principal branch
+  double az_now;                 // (t_now - t_span / 2) v_ground
+  double dxdy;                   // float32(dx) * float32(dy) as the reference multiplies
+};
+
+BraggConst bragg_const(const osar_bragg_desc& d) {
+  BraggConst c{};
+  const double U = d.wind_U;
   const double k_0 = kG / (U * U);
-  const double X = k_0 * fetch;
-  const double Omega_c = 0.84 * pow(tanh(pow(X / kX0, 0.4)), -0.75);
-  const double cK = sqrt(kG / k + kSurfT / kRho * k);
+  const double X = k_0 * d.wind_fetch;
+  const double Omega_c = 0.84 * std::pow(std::tanh(std::pow(X / kX0, 0.4)), -0.75);
   const double k_p = k_0 * Omega_c * Omega_c;
-  const double cK_p = sqrt(kG / k_p + kSurfT / kRho * k_p);
-  const double k_m = sqrt(kRho * kG / kSurfT);
-  const double cK_m = sqrt(kG / k_m + kSurfT / kRho * k_m);
+  c.cK_p = std::sqrt(kG / k_p + kSurfT / kRho * k_p);
+  const double k_m = std::sqrt(kRho * kG / kSurfT);
+  c.cK_m = std::sqrt(kG / k_m + kSurfT / kRho * k_m);
   const double C_10 = (0.8 + 0.065 * U) * 1e-3;
-  const double ustar = sqrt(C_10) * U;
-  const double a_0 = log(2.) / 4.;
-  const double a_m = 0.13 * ustar / cK_m;
-  return tanh(a_0 + 4. * pow(cK / cK_p, 2.5) + a_m * pow(cK_m / cK, 2.5));
+  const double ustar = std::sqrt(C_10) * U;
+  c.a_0 = std::log(2.) / 4.;
+  c.a_m = 0.13 * ustar / c.cK_m;
+  c.cos_wd = std::cos(d.wind_dir);
+  c.sin_wd = std::sin(d.wind_dir);
+  const double eps_abs = std::hypot(73., 18.);
+  c.sq_re = std::sqrt((eps_abs + 73.) / 2.);
+  c.sq_im = 18. / (2. * c.sq_re);
+  c.az_now = (d.pulse * d.t_step - d.t_span / 2.) * d.v_ground;
+  c.dxdy = (double)(float)d.dx * (double)(float)d.dy;
+  return c;
 }
 
-__device__ double spread_g(int model, double k, double th, double delta) {
-  if (model == 0) {
-    return (th > -M_PI / 2. && th < M_PI / 2.) ? (1. + delta * cos(2. * th)) / M_PI : 0.;
+// x^2.5 as x^2 sqrt(x) (the same value as pow(x, 2.5) to rounding)
+__device__ __forceinline__ double pow25(double x) { return x * x * sqrt(x); }
+
+// (cos, sin) of atan(t): (1, t) / sqrt(1 + t^2), written so that |t| -> inf
+// gives (0, sign t) as atan(+-inf) = +-pi/2 does.
+__device__ __forceinline__ void cos_sin_atan(double t, double* c, double* s) {
+  if (fabs(t) <= 1.) {
+    const double r = rsqrt(1. + t * t);
+    *c = r;
+    *s = t * r;
+  } else {
+    const double u = 1. / t;                       // |u| < 1, 0 for t = +-inf
+    const double r = rsqrt(1. + u * u);
+    *c = fabs(u) * r;
+    *s = copysign(r, t);
   }
-  const double c = cos(th);
-  return (th < -M_PI / 2. || th > M_PI / 2.) ? 0. : 1. / M_PI * c * c;
 }
 
-// scipy interp1d linear with fill_value=1 (searchsorted side='left').
-__device__ double interp_fill1(double x, const double* xp, const double* fp, int n) {
-  if (x < xp[0] || x > xp[n - 1]) return 1.0;
+// a / (d d) for complex a, d as a conj(d d) / |d d|^2: one real division
+__device__ __forceinline__ cd cdiv_sq(cd a, cd den) {
+  const cd q = cmulc(den, den);
+  const double inv = 1. / (q.re * q.re + q.im * q.im);
+  return cscale(cmulc(a, {q.re, -q.im}), inv);
+}
+
+// Grid point i of a table built as numpy.arange(x0, ., dk): x0 + i * dk with
+// the product rounded before the sum, as numpy fills it.
+__device__ __forceinline__ double arange_point(double x0, double dk, int i) {
+#pragma clang fp contract(off)
+  return x0 + (double)i * dk;
+}
+
+// Table point i: computed when the table is an exact arange grid (no load on
+// the search path), read otherwise.
+template <bool UNIFORM>
+__device__ __forceinline__ double table_x(const double* xp, double x0, double dk, int i) {
+  return UNIFORM ? arange_point(x0, dk, i) : xp[i];
+}
+
+// scipy interp1d linear with fill_value=1. The first index with xp[i] >= x
+// (searchsorted side='left') is found from a guess on the table's mean step and
+// corrected by a local walk, so any ascending table gives the exact index.
+template <bool UNIFORM>
+__device__ double interp_fill1(double x, const double* xp, const double* fp, int n, double x0,
+                               double dk, double inv_dk) {
+  if (x < table_x<UNIFORM>(xp, x0, dk, 0) || x > table_x<UNIFORM>(xp, x0, dk, n - 1)) return 1.0;
   if (x != x) return x;  // NaN propagates, as in scipy
-  int lo = 0, hi = n;  // first index with xp[i] >= x
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (xp[mid] < x) lo = mid + 1; else hi = mid;
-  }
-  int idx = lo < 1 ? 1 : (lo > n - 1 ? n - 1 : lo);
-  const double xl = xp[idx - 1], xh = xp[idx];
+  int i = (int)fmin(fmax(ceil((x - x0) * inv_dk), 0.), (double)(n - 1));
+  while (i < n - 1 && table_x<UNIFORM>(xp, x0, dk, i) < x) ++i;
+  while (i > 0 && table_x<UNIFORM>(xp, x0, dk, i - 1) >= x) --i;
+  const int idx = i < 1 ? 1 : i;
+  const double xl = table_x<UNIFORM>(xp, x0, dk, idx - 1);
+  const double xh = table_x<UNIFORM>(xp, x0, dk, idx);
   const double yl = fp[idx - 1], yh = fp[idx];
   const double slope = (yh - yl) / (xh - xl);
   return slope * (x - xl) + yl;
 }
 
-__global__ __launch_bounds__(256) void facet_bragg_kernel(osar_bragg_desc d) {
+constexpr int kThreads = 256;
+
+// Every angle of romeiser97.py:262-314 enters only
+// through its sine and cosine, so atan/asin/sin/cos are replaced by the exact
+// identities cos(atan t) = 1/sqrt(1+t^2), cos(asin x) = sqrt(1-x^2) and angle
+// sums, and the wrapped spreading angle th of the two Bragg components by
+// cos(th) (th and th + pi have opposite cosines). The half-plane test
+// |th| < pi/2 becomes cos(th) > 0 (cos(th) < 0 for the cos^2 model's zero).
+// Grid-stride over facets; the per-thread RCS sums are reduced per workgroup
+// and added to one of kSlots partial sums (spreading the atomics over
+// addresses), which bragg_fold_kernel folds into rcs_sum.
+template <bool UNIFORM>
+__global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d, BraggConst c,
+                                                               double* slots) {
   const size_t n = (size_t)d.nx * d.ny;
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int npol = d.pol == 2 ? 2 : 1;
   double sums[2] = {0., 0.};
-  if (idx < n) {
+  for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n;
+       idx += (size_t)gridDim.x * kThreads) {
     // fields are x-major: idx = col * ny + row
     const int col = (int)(idx / d.ny);
     const int row = (int)(idx - (size_t)col * d.ny);
-    const double t_now = d.pulse * d.t_step;
-    const double az_now = (t_now - d.t_span / 2.) * d.v_ground;
-    const double az = asin((d.y_row[row] - az_now) / d.gr0);
-    const double inc = d.inc_col[col];
+    // azimuth angle az = asin(arg) of the row (sar_raw_nompi.py:363)
+    const double arg = (d.y_row[row] - c.az_now) / d.gr0;
+    const double sin_az = fabs(arg) <= 1. ? arg : NAN;
+    const double cos_az = sqrt(1. - arg * arg);
+    const double cos_inc = d.cos_inc[col], sin_inc = d.sin_inc[col];
     const double diffx = (double)d.diffx[idx];
     const double diffy = (double)d.diffy[idx];
-    // romeiser97.py:262-314
-    const double cos_az = cos(az), sin_az = sin(az);
-    const double s_p = atan(cos_az * diffx + sin_az * diffy);
-    const double s_t = atan(-sin_az * diffx + cos_az * diffy);
-    const double cos_s_t = cos(s_t);
-    const double cos_s_t_2 = cos_s_t * cos_s_t;
+    // parallel and orthogonal slopes s_p = atan(u), s_t = atan(v)
+    double cos_sp, sin_sp, cos_st, sin_st;
+    cos_sin_atan(cos_az * diffx + sin_az * diffy, &cos_sp, &sin_sp);
+    cos_sin_atan(-sin_az * diffx + cos_az * diffy, &cos_st, &sin_st);
+    const double cos_s_t_2 = cos_st * cos_st;
     const double sin_s_t_2 = 1 - cos_s_t_2;
-    const double sin_ip = sin(inc - s_p);
-    const double cos_ip = cos(inc - s_p);
+    const double sin_ip = sin_inc * cos_sp - cos_inc * sin_sp;   // sin(inc - s_p)
+    const double cos_ip = cos_inc * cos_sp + sin_inc * sin_sp;   // cos(inc - s_p)
     const double cos_ip_2 = cos_ip * cos_ip;
     const double sin_ip_2 = sin_ip * sin_ip;
-    const double cos_ti = cos_ip * cos_s_t;
+    const double cos_ti = cos_ip * cos_st;
     const double sin_ti_2 = 1 - cos_ti * cos_ti;
     const cd eps = {73., 18.};
-    // sqrt(73 + 18j), principal branch
-    const double eps_abs = hypot(73., 18.);
-    const cd sq = {sqrt((eps_abs + 73.) / 2.), 18. / (2. * sqrt((eps_abs + 73.) / 2.))};
-    const cd den_h = cadd({cos_ti, 0.}, sq);
-    const cd b_hh = cdiv(eps, cmulc(den_h, den_h));
-    const cd den_v = cadd(cscale(eps, cos_ti), sq);
-    const cd b_vv = cdiv(cscale(cmulc(eps, eps), 1. + sin_ti_2), cmulc(den_v, den_v));
-    const double wgt = cos_ip / (cos(inc) * cos(s_p));
-    const double a = sin_s_t_2 / sin_ti_2;
-    const double b = sin_ip_2 * cos_s_t_2 / sin_ti_2;
+    const cd sq = {c.sq_re, c.sq_im};
+    const cd b_hh = cdiv_sq(eps, cadd({cos_ti, 0.}, sq));
+    const cd b_vv = cdiv_sq(cscale(cmulc(eps, eps), 1. + sin_ti_2), cadd(cscale(eps, cos_ti), sq));
+    const double wgt = cos_ip / (cos_inc * cos_sp);
+    const double inv_ti = 1. / sin_ti_2;
+    const double a = sin_s_t_2 * inv_ti;
+    const double b = sin_ip_2 * cos_s_t_2 * inv_ti;
     const double kc = d.k0 * cos_ti;
     const double F0 = 8 * M_PI * (kc * kc) * (kc * kc);
     const double T_vv = wgt * (F0 * cabs2(cadd(cscale(b_vv, b), cscale(b_hh, a))));
     const double T_hh = wgt * (F0 * cabs2(cadd(cscale(b_hh, b), cscale(b_vv, a))));
     double k_b = 2 * d.k0 * sqrt(sin_ip_2 + cos_ip_2 * sin_s_t_2);
     if (k_b > 2 * d.k0) k_b = 0.;
-    const double phi_b = az + atan((cos_ip * sin(-s_t)) / sin(inc - s_p));
-    double th[2];
-    th[0] = phi_b - d.wind_dir;
-    th[1] = phi_b + M_PI - d.wind_dir;
-    th[0] = atan2(sin(th[0]), cos(th[0]));  // np.angle(np.exp(1j * th))
-    th[1] = atan2(sin(th[1]), cos(th[1]));
+    // phi_b = az + atan(q); th0 = phi_b - wind_dir; cos(th0) from angle sums
+    double cos_q, sin_q;
+    cos_sin_atan((cos_ip * -sin_st) / sin_ip, &cos_q, &sin_q);
+    const double cos_A = cos_az * c.cos_wd + sin_az * c.sin_wd;  // A = az - wind_dir
+    const double sin_A = sin_az * c.cos_wd - cos_az * c.sin_wd;
+    const double cth0 = cos_A * cos_q - sin_A * sin_q;
     const bool bad = 4 * sin_ip_2 < d.d * d.d;
-    const double S1 = interp_fill1(k_b, d.spec_k, d.spec_S, d.n_spec);
-    const double delta = d.spread == 0 ? elf_delta(k_b, d.wind_U, d.wind_fetch) : 0.;
-    const double dxf = (double)(float)d.dx, dyf = (double)(float)d.dy;
-    for (int c = 0; c < 2; ++c) {
-      const double spectrum = S1 / k_b * spread_g(d.spread, k_b, th[c], delta);
+    const double S1 = interp_fill1<UNIFORM>(k_b, d.spec_k, d.spec_S, d.n_spec, d.spec_k0,
+                                            d.spec_dk, d.spec_inv_dk);
+    double delta = 0.;
+    if (d.spread == 0) {
+      const double cK = sqrt(kG / k_b + kSurfT / kRho * k_b);
+      delta = tanh(c.a_0 + 4. * pow25(cK / c.cK_p) + c.a_m * pow25(c.cK_m / cK));
+    }
+    const double base = S1 / k_b;
+    for (int comp = 0; comp < 2; ++comp) {
+      const double cth = comp == 0 ? cth0 : -cth0;
+      double g;
+      constexpr double kInvPi = 0.31830988618379067154;
+      if (d.spread == 0) g = cth > 0. ? (1. + delta * (2. * cth * cth - 1.)) * kInvPi : 0.;
+      else g = cth < 0. ? 0. : kInvPi * cth * cth;
+      const double spectrum = base * g;
       // pol 2 (DP): slot 0 = hh, slot 1 = vv
       for (int ip = 0; ip < npol; ++ip) {
         const bool is_hh = (d.pol == 1) || (d.pol == 2 && ip == 0);
         const double T = is_hh ? T_hh : T_vv;
-        double rcs = T * spectrum * dxf * dyf;
+        double rcs = T * spectrum * c.dxdy;
         // romeiser97.py:366-392 (DP: hh masks only the plus component)
-        const bool mask = bad && !(d.pol == 2 && is_hh && c == 0);
+        const bool mask = bad && !(d.pol == 2 && is_hh && comp == 0);
         if (mask) rcs = 0.;
-        d.amp[((size_t)ip * 2 + c) * n + idx] = (float)sqrt(rcs);
+        d.amp[((size_t)ip * 2 + comp) * n + idx] = (float)sqrt(rcs);
         sums[ip] += rcs;
       }
     }
   }
-  // block reduction of the RCS sums
-  __shared__ double red[2][256];
-  for (int ip = 0; ip < 2; ++ip) red[ip][threadIdx.x] = sums[ip];
+  // wave, then workgroup reduction of the RCS sums
+  __shared__ double red[kThreads / 64][2];
+  for (int ip = 0; ip < 2; ++ip)
+    for (int off = 32; off > 0; off >>= 1) sums[ip] += __shfl_xor(sums[ip], off);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wave][0] = sums[0];
+    red[wave][1] = sums[1];
+  }
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + s];
-      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+  if (threadIdx.x == 0 && slots) {
+    for (int ip = 0; ip < npol; ++ip) {
+      double v = 0.;
+      for (int w = 0; w < kThreads / 64; ++w) v += red[w][ip];
+      atomicAdd(&slots[(blockIdx.x % kBraggSlots) * 2 + ip], v);
     }
-    __syncthreads();
   }
-  if (threadIdx.x == 0 && d.rcs_sum) {
-    for (int ip = 0; ip < npol; ++ip) atomicAdd(&d.rcs_sum[ip], red[ip][0]);
+}
+
+// Adds the partial sums to rcs_sum and clears them for the next call (a
+// kernel, not a memset, so that a captured graph replays it correctly).
+__global__ void bragg_fold_kernel(double* slots, double* rcs_sum, int npol) {
+  const int t = threadIdx.x;  // one wave, kBraggSlots == 64
+  for (int ip = 0; ip < npol; ++ip) {
+    double v = slots[t * 2 + ip];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (t == 0) rcs_sum[ip] += v;
   }
+  slots[t * 2] = 0.;
+  slots[t * 2 + 1] = 0.;
 }
 
 }  // namespace
@@ -177,7 +252,8 @@ extern "C" int osar_facet_bragg(osar_ctx* ctx, const osar_bragg_desc* d, void* s
   OSAR_REQUIRE(d->spread == 0 || d->spread == 1, OSAR_EUNSUPPORTED,
                "spread must be 0 (elfouhaily) or 1 (cos2s)");
   OSAR_REQUIRE(d->n_spec >= 2, OSAR_EINVAL, "spectrum table needs >= 2 points");
-  OSAR_PTR(d->inc_col);
+  OSAR_PTR(d->cos_inc);
+  OSAR_PTR(d->sin_inc);
   OSAR_PTR(d->y_row);
   OSAR_PTR(d->spec_k);
   OSAR_PTR(d->spec_S);
@@ -185,9 +261,25 @@ extern "C" int osar_facet_bragg(osar_ctx* ctx, const osar_bragg_desc* d, void* s
   OSAR_PTR(d->diffy);
   OSAR_PTR(d->amp);
   OSAR_HIP(hipSetDevice(ctx->device));
+  OSAR_REQUIRE(!d->spec_uniform || d->spec_dk > 0., OSAR_EINVAL,
+               "spec_uniform needs spec_dk > 0");
   const size_t n = (size_t)d->nx * d->ny;
-  const unsigned blocks = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(facet_bragg_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, *d);
+  size_t want = (n + kThreads - 1) / kThreads;
+  const size_t cap = (size_t)ctx->n_cu * 8;   // grid-stride beyond 8 workgroups per CU
+  const unsigned blocks = (unsigned)(want < cap ? want : cap);
+  const BraggConst c = bragg_const(*d);
+  double* slots = d->rcs_sum ? ctx->bragg_slots : nullptr;
+  if (d->spec_uniform)
+    hipLaunchKernelGGL(facet_bragg_kernel<true>, dim3(blocks), dim3(kThreads), 0,
+                       (hipStream_t)stream, *d, c, slots);
+  else
+    hipLaunchKernelGGL(facet_bragg_kernel<false>, dim3(blocks), dim3(kThreads), 0,
+                       (hipStream_t)stream, *d, c, slots);
   OSAR_LAUNCHED("facet_bragg_kernel");
+  if (slots) {
+    hipLaunchKernelGGL(bragg_fold_kernel, dim3(1), dim3(kBraggSlots), 0, (hipStream_t)stream,
+                       slots, d->rcs_sum, d->pol == 2 ? 2 : 1);
+    OSAR_LAUNCHED("bragg_fold_kernel");
+  }
   return OSAR_OK;
 }

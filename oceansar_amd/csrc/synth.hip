@@ -53,13 +53,13 @@ struct QPlan {
 };
 
 struct SynthArgs {
-  int nx, ny, nq, logx, logy, cb;
+  int nx, ny, nq, logx, logy, cb, lcb;
   double depth, cur_x, cur_y, thr;
   const float* kx;
   const float* ky;
   const osar_c32* coefs;
   const double* times;
-  const double2* twx;
+  const double2* twx;   // [128]: exp(2 pi i k / nx) for k < 64, then for k = 64 m
   const double2* twy;
   double2* work;
   float* out[8];
@@ -97,12 +97,30 @@ __device__ __forceinline__ double2 herm(int m, double kx, double ky, double ik, 
   return make_double2(0.5 * (g.x + gm.x), 0.5 * (g.y - gm.y));
 }
 
+// Twiddle exp(+2 pi i k / N) from the two-level LDS table: tw[k & 63] for the
+// low six bits times tw[64 + (k >> 6)] for the rest (one FP64 complex product,
+// within 2 ulp of the directly rounded value).
+__device__ __forceinline__ double2 twiddle(const double2* tw, int k) {
+  const double2 lo = tw[k & 63];
+  if (k < 64) return lo;
+  return cmul2(lo, tw[64 + (k >> 6)]);
+}
+
+// Copy the 128-entry two-level twiddle table of a transform length to LDS.
+__device__ __forceinline__ void load_twiddles(double2* dst, const double2* src) {
+  for (int i = threadIdx.x; i < 128; i += blockDim.x) dst[i] = src[i];
+}
+
 // In-LDS inverse FFT of nseq power-of-two sequences (Stockham, radix 4 with a
 // final radix-2 stage when log2 N is odd). Element e of sequence s lives at
-// buf[s * ss + e * es]. tw[k] = exp(+2 pi i k / N). Results stay in buf.
-template <int NT, int BPT>
-__device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int ss, int es,
-                         const double2* __restrict__ tw) {
+// buf[s * ss + e * es]; tw is the two-level twiddle table in LDS. Results stay
+// in buf. SEQ_FAST: consecutive threads take consecutive sequences (nseq a
+// power of two, log2 given), for interleaved layouts (es = nseq, ss = 1) where
+// that keeps every LDS access of a wave contiguous; otherwise consecutive
+// threads take consecutive butterflies of one sequence.
+template <int NT, int BPT, bool SEQ_FAST>
+__device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int lseq, int ss, int es,
+                         const double2* tw) {
   const int tid = threadIdx.x;
   int Ns = 1, logNs = 0;
   int left = logN;
@@ -117,7 +135,8 @@ __device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int ss, int es
     for (int b = 0; b < BPT; ++b) {
       const int id = tid + b * NT;
       if (id < total) {
-        const int s = id >> lnb, j = id & (nb - 1);
+        const int s = SEQ_FAST ? (id & (nseq - 1)) : (id >> lnb);
+        const int j = SEQ_FAST ? (id >> lseq) : (id & (nb - 1));
 #pragma unroll
         for (int r = 0; r < 4; ++r)
           if (r < R) v[b][r] = buf[s * ss + (j + r * nb) * es];
@@ -128,14 +147,15 @@ __device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int ss, int es
     for (int b = 0; b < BPT; ++b) {
       const int id = tid + b * NT;
       if (id < total) {
-        const int s = id >> lnb, j = id & (nb - 1);
+        const int s = SEQ_FAST ? (id & (nseq - 1)) : (id >> lnb);
+        const int j = SEQ_FAST ? (id >> lseq) : (id & (nb - 1));
         const int jm = j & (Ns - 1);
         const int step = jm << (logN - logNs - lr);   // jm * N / (Ns * R)
         if (R == 4) {
           const double2 x0 = v[b][0];
-          const double2 x1 = cmul2(v[b][1], tw[step]);
-          const double2 x2 = cmul2(v[b][2], tw[2 * step]);
-          const double2 x3 = cmul2(v[b][3], tw[3 * step]);
+          const double2 x1 = cmul2(v[b][1], twiddle(tw, step));
+          const double2 x2 = cmul2(v[b][2], twiddle(tw, 2 * step));
+          const double2 x3 = cmul2(v[b][3], twiddle(tw, 3 * step));
           const double2 s02 = make_double2(x0.x + x2.x, x0.y + x2.y);
           const double2 d02 = make_double2(x0.x - x2.x, x0.y - x2.y);
           const double2 s13 = make_double2(x1.x + x3.x, x1.y + x3.y);
@@ -148,7 +168,7 @@ __device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int ss, int es
           buf[s * ss + (o + 3 * Ns) * es] = make_double2(d02.x + d13.y, d02.y - d13.x);
         } else {
           const double2 x0 = v[b][0];
-          const double2 x1 = cmul2(v[b][1], tw[step]);
+          const double2 x1 = cmul2(v[b][1], twiddle(tw, step));
           const int o = (j - jm) * 2 + jm;
           buf[s * ss + o * es] = make_double2(x0.x + x1.x, x0.y + x1.y);
           buf[s * ss + (o + Ns) * es] = make_double2(x0.x - x1.x, x0.y - x1.y);
@@ -175,8 +195,10 @@ constexpr int kRowPairs = kMaxN / kRowThreads;   // column pairs a row thread ow
 __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double2* S = (double2*)lds_raw;  // [2][nx] packed spectrum of rows j, jm
+  double2* tw = S + 2 * a.nx;      // [128] twiddles
+  load_twiddles(tw, a.twx);
   const int j = blockIdx.x;
-  const int jm = (a.ny - j) % a.ny;
+  const int jm = (a.ny - j) & (a.ny - 1);
   const int nrow = (j == jm) ? 1 : 2;
   const int ti = blockIdx.y;
   const double t = a.times[ti];
@@ -186,6 +208,7 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
   const double kyj = (double)a.ky[j], kyjm = (double)a.ky[jm];
   double2 A0[kRowPairs], A1[kRowPairs];   // a(k) at (j, i) and a(-k) at (jm, -i)
   double ik0[kRowPairs], ik1[kRowPairs];  // 1/|k| (0 off the good-k set)
+  double kx0[kRowPairs], kx1[kRowPairs];  // kx at columns i and -i
 #pragma unroll
   for (int u = 0; u < kRowPairs; ++u) {
     const int i = tid + u * kRowThreads;
@@ -205,9 +228,11 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
       if (side == 0) {
         A0[u] = v;
         ik0[u] = ik;
+        kx0[u] = kx;
       } else {
         A1[u] = v;
         ik1[u] = ik;
+        kx1[u] = kx;
       }
     }
   }
@@ -218,7 +243,7 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
       const int i = tid + u * kRowThreads;
       if (i >= nx) break;
       const int im = (nx - i) & (nx - 1);
-      const double kx = (double)a.kx[i], kxm = (double)a.kx[im];
+      const double kx = kx0[u], kxm = kx1[u];
       // S at k = (j, i)
       const double2 ha = herm(q.ma, kx, kyj, ik0[u], kxm, kyjm, ik1[u], A0[u], A1[u]);
       const double2 hb = herm(q.mb, kx, kyj, ik0[u], kxm, kyjm, ik1[u], A0[u], A1[u]);
@@ -229,12 +254,12 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
       S[(nrow - 1) * nx + im] = make_double2(hc.x - hd.y, hc.y + hd.x);
     }
     __syncthreads();
-    lds_ifft<kRowThreads, 4>(S, nx, a.logx, nrow, nx, 1, a.twx);
+    lds_ifft<kRowThreads, 4, false>(S, nx, a.logx, nrow, 0, nx, 1, tw);
     double2* w = a.work + (size_t)(ti * a.nq + qi) * a.ny * nx;
     for (int e = tid; e < nrow * nx; e += kRowThreads) {
-      const int r = e / nx, i = e - r * nx;
+      const int r = e >> a.logx, i = e & (nx - 1);
       const int row = r == 0 ? j : jm;
-      const int xb = i / cb, c = i - xb * cb;
+      const int xb = i >> a.lcb, c = i & (cb - 1);
       w[((size_t)xb * a.ny + row) * cb + c] = S[e];
     }
     __syncthreads();
@@ -246,8 +271,10 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
 // scaled by 1/(Nx Ny) and written as float32 fields, x-major (field[x][y]).
 __global__ __launch_bounds__(kColThreads) void synth_cols_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  double2* C = (double2*)lds_raw;  // [ny][cb]
   const int cb = a.cb;
+  double2* C = (double2*)lds_raw;  // [ny][cb]
+  double2* tw = C + (size_t)a.ny * cb;  // [128] twiddles
+  load_twiddles(tw, a.twy);
   const int xb = blockIdx.x;
   const int x0 = xb * cb;
   const int qi = blockIdx.y;
@@ -257,13 +284,13 @@ __global__ __launch_bounds__(kColThreads) void synth_cols_kernel(SynthArgs a) {
   const double2* w = a.work + (size_t)(ti * a.nq + qi) * ny * nx + (size_t)xb * ny * cb;
   for (int e = tid; e < ny * cb; e += kColThreads) C[e] = w[e];
   __syncthreads();
-  lds_ifft<kColThreads, 8>(C, ny, a.logy, cb, 1, cb, a.twy);
+  lds_ifft<kColThreads, 8, true>(C, ny, a.logy, cb, a.lcb, 1, cb, tw);
   const QPlan q = a.q[qi];
   const double scale = 1.0 / ((double)nx * (double)ny);
   const double t = a.times[ti];
   const size_t plane = (size_t)nx * ny;
   for (int e = tid; e < ny * cb; e += kColThreads) {
-    const int c = e / ny, y = e - c * ny;  // consecutive threads walk y: contiguous stores
+    const int c = e >> a.logy, y = e & (ny - 1);  // consecutive threads walk y: contiguous stores
     const double2 z = C[y * cb + c];
     const double re = z.x * scale, im = z.y * scale;
     const size_t o = (size_t)ti * plane + (size_t)(x0 + c) * ny + y;
@@ -325,7 +352,8 @@ struct TwCache {
   double2* dev = nullptr;
 };
 
-// Twiddle tables exp(+2 pi i k / N), computed on the host in long double.
+// Two-level twiddle tables of exp(+2 pi i k / N), computed on the host in long
+// double: entries 0..63 hold k = 0..63, entries 64..127 hold k = 64 m.
 int twiddles(osar_ctx* ctx, int n, const double2** out) {
   static thread_local TwCache cache[64][16];
   const int l = ilog2(n);
@@ -333,15 +361,16 @@ int twiddles(osar_ctx* ctx, int n, const double2** out) {
                "twiddle table for device %d, n %d", ctx->device, n);
   TwCache& c = cache[ctx->device][l];
   if (c.n != n || c.dev == nullptr) {
-    std::vector<double2> h(n);
+    std::vector<double2> h(128);
     const long double pi = 3.14159265358979323846264338327950288L;
-    for (int k = 0; k < n; ++k) {
-      const long double ang = 2.0L * pi * (long double)k / (long double)n;
-      h[k] = make_double2((double)cosl(ang), (double)sinl(ang));
+    for (int e = 0; e < 128; ++e) {
+      const long k = e < 64 ? e : 64L * (e - 64);
+      const long double ang = 2.0L * pi * (long double)(k % n) / (long double)n;
+      h[e] = make_double2((double)cosl(ang), (double)sinl(ang));
     }
     double2* p = nullptr;
-    OSAR_HIP(hipMalloc(&p, sizeof(double2) * n));
-    OSAR_HIP(hipMemcpy(p, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    OSAR_HIP(hipMalloc(&p, sizeof(double2) * 128));
+    OSAR_HIP(hipMemcpy(p, h.data(), sizeof(double2) * 128, hipMemcpyHostToDevice));
     c.n = n;
     c.dev = p;
   }
@@ -409,9 +438,10 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   if (cb > 8) cb = 8;
   if (cb > d->nx) cb = d->nx;
   a.cb = cb;
+  a.lcb = ilog2(cb);
   hipStream_t s = (hipStream_t)stream;
-  const size_t lds_rows = 2 * (size_t)d->nx * sizeof(double2);
-  const size_t lds_cols = (size_t)cb * d->ny * sizeof(double2);
+  const size_t lds_rows = (2 * (size_t)d->nx + 128) * sizeof(double2);
+  const size_t lds_cols = ((size_t)cb * d->ny + 128) * sizeof(double2);
   OSAR_HIP(hipFuncSetAttribute((const void*)synth_rows_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows));
   OSAR_HIP(hipFuncSetAttribute((const void*)synth_cols_kernel,

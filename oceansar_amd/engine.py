@@ -60,6 +60,20 @@ def synth_fields(ctx, surf, kx, ky, coefs, times, out, fields=None):
     return times_t
 
 
+def arange_grid(k):
+    """(k0, dk, 1) when k[i] == k0 + i * dk exactly for every i, with the product
+    rounded before the sum (numpy.arange fills its output that way), so the Bragg
+    kernel may compute grid points instead of loading them; (k0, 0, 0) otherwise."""
+    k = np.asarray(k, dtype=np.float64)
+    if k.size < 2:
+        return (float(k[0]) if k.size else 0.), 0., 0
+    dk = k[1] - k[0]
+    exact = k[0] + np.arange(k.size, dtype=np.float64) * dk
+    if dk > 0 and np.array_equal(exact, k):
+        return float(k[0]), float(dk), 1
+    return float(k[0]), 0., 0
+
+
 class LaunchShape:
     """Echo-accumulator launch shape: pulses per workgroup, rows per workgroup, LDS window."""
 
@@ -99,6 +113,7 @@ class DeviceScene:
             self.ind0_p = _t(np.mod(sc.scat_p.ind0, sc.scat_p.N).T, i32, dev)
             self.rcs_slot = _t(sc.rcs_slot, i32, dev)
             self.spec_k = _t(sc.bragg.spec_k, f64, dev)
+            self.spec_grid = arange_grid(sc.bragg.spec_k)
             self.spec_S = _t(sc.bragg.spec_S, f64, dev)
         surf = sc.surface
         self.coefs = _t(surf.wave_coefs, c64, dev)
@@ -180,8 +195,12 @@ class DeviceScene:
         d.k0, d.d, d.wind_dir, d.wind_U, d.wind_fetch = b.k0, b.d, b.wind_dir, b.wind_U, b.wind_fetch
         d.dx, d.dy = float(sc.surface.dx), float(sc.surface.dy)
         d.t_step, d.t_span, d.v_ground, d.gr0 = sc.t_step, sc.t_span, sc.v_ground, sc.gr0
-        d.inc_col, d.y_row = _lib.ptr(self.inc_col), _lib.ptr(self.y_row)
+        d.cos_inc, d.sin_inc = _lib.ptr(self.cos_inc), _lib.ptr(self.sin_inc)
+        d.y_row = _lib.ptr(self.y_row)
         d.spec_k, d.spec_S = _lib.ptr(self.spec_k), _lib.ptr(self.spec_S)
+        d.spec_k0, d.spec_dk, d.spec_uniform = self.spec_grid
+        k = b.spec_k
+        d.spec_inv_dk = float((len(k) - 1) / (k[-1] - k[0])) if len(k) > 1 else 0.
         d.diffx, d.diffy = _lib.ptr(diffx), _lib.ptr(diffy)
         d.amp = _lib.ptr(amp_out)
         d.rcs_sum = _lib.ptr(rcs_sum_out)

@@ -42,26 +42,46 @@ def _snapshots(sc, m0, m1):
     return snaps, out
 
 
-def test_bragg_kernel_matches_oracle(need_gpu, cfg1_scene):
+def _bragg_compare(sc, m):
     import torch
     import oracle.sar as osar
     from oceansar_amd.engine import DeviceScene
-    sc = cfg1_scene
     ds = DeviceScene(sc)
-    snaps, dev = _snapshots(sc, 25, 25)
-    p = 25 * sc.K
-    amp = torch.empty((1, 2, sc.Nx, sc.Ny), dtype=torch.float32, device='cuda')
-    tot = torch.zeros(1, dtype=torch.float64, device='cuda')
+    snaps, dev = _snapshots(sc, m, m)
+    p = m * sc.K
+    npol = len(sc.pols)
+    amp = torch.empty((npol, 2, sc.Nx, sc.Ny), dtype=torch.float32, device='cuda')
+    tot = torch.zeros(npol, dtype=torch.float64, device='cuda')
     ds.bragg(dev['Diffx'][0], dev['Diffy'][0], p, amp, tot)
     cache = osar.SnapshotCache(sc)
-    cache.cache[25] = snaps[0]
-    rcs = osar.bragg_rcs(sc, cache, p)['vv']
-    ref = np.sqrt(rcs)
-    got = amp[0].cpu().numpy().transpose(0, 2, 1)
-    assert np.all(np.isfinite(got))
-    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)
-    assert np.max(err[ref > 1e-12]) < 2e-6
-    assert abs(tot.item() - rcs.sum()) / rcs.sum() < 1e-10
+    cache.cache[m] = snaps[0]
+    rcs_all = osar.bragg_rcs(sc, cache, p)
+    for ip, pol in enumerate(sc.pols):
+        rcs = rcs_all[pol]
+        ref = np.sqrt(rcs)
+        got = amp[ip].cpu().numpy().transpose(0, 2, 1)
+        assert np.all(np.isfinite(got))
+        err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30)
+        # float32 rounding of sqrt(RCS); the RCS itself is FP64
+        assert np.max(err[ref > 1e-12]) < 2e-6, pol
+        assert np.all(got[ref <= 1e-12] < 1e-5), pol
+        assert abs(tot[ip].item() - rcs.sum()) / rcs.sum() < 1e-10, pol
+
+
+def test_bragg_kernel_matches_oracle(need_gpu, cfg1_scene):
+    """cos^2 spreading, vv."""
+    _bragg_compare(cfg1_scene, 25)
+
+
+@pytest.mark.parametrize('pol', ['vv', 'hh', 'DP'])
+def test_bragg_kernel_elfouhaily(need_gpu, tmp_path_factory, pol):
+    """Elfouhaily spreading (the Delta(k) terms), every polarisation including the
+    DP mask quirk, wind across the scene so both half-planes are populated."""
+    from oceansar_amd import configs
+    sc = configs.load_scene(configs.workload('small', L=128., spread='elfouhaily', pol=pol,
+                                             wind_dir=60., num_ch=1),
+                            tmp_path_factory.mktemp('be' + pol))
+    _bragg_compare(sc, 7)
 
 
 def test_accumulator_matches_oracle(need_gpu, cfg1_scene):

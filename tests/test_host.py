@@ -136,3 +136,17 @@ def test_c_oracle_matches_numpy_oracle(tmp_path):
     b, nb = co.simulate_c(sc, 300, 330, snaps=cache)
     assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-12
     np.testing.assert_allclose(na, nb, rtol=1e-12)
+
+
+def test_arange_grid_detection():
+    """The Bragg table search computes grid points only for exact numpy.arange grids."""
+    from oceansar_amd.engine import arange_grid
+    k = np.arange(1, 2 * 113.27, 0.1)
+    k0, dk, uni = arange_grid(k)
+    assert uni == 1 and k0 == 1.0 and dk == k[1] - k[0]
+    assert np.array_equal(k0 + np.arange(k.size) * dk, k)
+    bent = k.copy()
+    bent[17] = np.nextafter(bent[17], 0.)
+    assert arange_grid(bent)[2] == 0
+    assert arange_grid(np.linspace(1, 5, 9) ** 2)[2] == 0
+    assert arange_grid(np.array([3.]))[2] == 0
