@@ -21,11 +21,12 @@
 // straight to global memory; taps outside [0, nbins) are dropped as in the
 // reference.
 //
-// Precision: slant range, bin position, phases and the lerp of the
-// displacement fields are FP64 in the reference's operation order (FP
-// contraction off), so bin and sub-bin indices match an FP64 evaluation; the
-// phase is reduced to revolutions in FP64 before FP32 sin/cos; scene values,
-// channel factors and the accumulation are FP32.
+// Precision: the fields between snapshots are interpolated in float32 as the
+// reference's balancer does; slant range, bin position and phases are FP64 in
+// the reference's operation order (FP contraction off), so bin and sub-bin
+// indices match an FP64 evaluation; the phase is reduced to revolutions in
+// FP64 before the FP32 sin/cos; scene values, channel factors and the
+// accumulation are FP32 (LDS partial sums FP64).
 #include <limits.h>
 
 #include "common.h"
@@ -250,11 +251,15 @@ __device__ __forceinline__ void load_facet(const osar_echo_desc& d, int x, size_
   }
 }
 
-__device__ __forceinline__ double lerp2(float a, float b, int rk, double w) {
+// Field value between two snapshots as the reference's snapshot balancer
+// interpolates (oceansar/surfaces/balancer.py:274-282): float32 products and
+// sum, w_l * a + w_h * b (no fused multiply-add), widened to FP64 for the range.
+__device__ __forceinline__ double lerp2(float a, float b, int rk, float wl, float wh) {
 #pragma clang fp contract(off)
-  const double x = (double)a;
-  if (rk == 0) return x;
-  return (1. - w) * x + w * (double)b;
+  if (rk == 0) return (double)a;
+  const float x = wl * a;
+  const float y = wh * b;
+  return (double)(x + y);
 }
 
 // Range-line layout in LDS: line (pol, ch, pulse) at offset line * LS doubles,
@@ -383,7 +388,8 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
   const int pc = active ? p : pstart;
   const int m = pc / d.K;
   const int rk = pc - m * d.K;
-  const double w = (double)rk / (double)d.K;
+  const float wh = (float)((double)rk / (double)d.K);   // w_h, w_l rounded to float32
+  const float wl = (float)(1.0 - (double)wh);
   const size_t plane = (size_t)d.nx * d.ny;
   const size_t snap_off = (size_t)(m - d.snap0) * plane;
   const bool bragg = (d.flags & OSAR_ECHO_BRAGG) != 0;
@@ -446,9 +452,9 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
         load_facet<NPOL, SPEC, XTI>(d, x + 1, idx + d.ny, snap_off + idx + d.ny, plane, rk, p,
                                     rslot, bragg_ok, ib, nxt);
       }
-      const double Dx = lerp2(cur.dx0, cur.dx1, rk, w);
-      const double Dy = lerp2(cur.dy0, cur.dy1, rk, w);
-      const double Dz = lerp2(cur.dz0, cur.dz1, rk, w);
+      const double Dx = lerp2(cur.dx0, cur.dx1, rk, wl, wh);
+      const double Dy = lerp2(cur.dy0, cur.dy1, rk, wl, wh);
+      const double Dz = lerp2(cur.dz0, cur.dz1, rk, wl, wh);
       const Geo g = facet_range(cur.sr, cur.ci, cur.si, mig, sin_az, Dx, Dy,
                                 Dz, d.sr0);
       const double xb = osar::div_by(g.srs - d.min_sr, d.binsize, inv_binsize);
@@ -476,11 +482,11 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
       if (SPEC) {
         // Esn = exp(-i 2 k0 sr_surface) (4 pi)^1.5 E   (sar_raw_nompi.py:405-426)
         zd e[2];
-        if (finite && ka_spa(d.k0, spol, cur.inc, phi, lerp2(cur.s0[0], cur.s1[0], rk, w),
-                             lerp2(cur.s0[1], cur.s1[1], rk, w),
-                             lerp2(cur.s0[2], cur.s1[2], rk, w),
-                             lerp2(cur.s0[3], cur.s1[3], rk, w),
-                             lerp2(cur.s0[4], cur.s1[4], rk, w), d.facet_dx, d.facet_dy, e)) {
+        if (finite && ka_spa(d.k0, spol, cur.inc, phi, lerp2(cur.s0[0], cur.s1[0], rk, wl, wh),
+                             lerp2(cur.s0[1], cur.s1[1], rk, wl, wh),
+                             lerp2(cur.s0[2], cur.s1[2], rk, wl, wh),
+                             lerp2(cur.s0[3], cur.s1[3], rk, wl, wh),
+                             lerp2(cur.s0[4], cur.s1[4], rk, wl, wh), d.facet_dx, d.facet_dy, e)) {
           double ps, pcs;
           sincos(-(2. * d.k0 * g.srs), &ps, &pcs);
           const double c15 = 44.546623974653535;  // (4 pi)^1.5

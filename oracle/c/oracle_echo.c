@@ -6,7 +6,8 @@
  * timed CPU baseline (a stand-in for the reference's numba nopython loop).
  *
  * Per pulse p (one OpenMP thread per pulse, so output lines are private):
- *   fields      (1-w) S[m] + w S[m+1], w = (p mod K)/K      balancer.py:274-297
+ *   fields      w_l S[m] + w_h S[m+1] in float32, w_h = (p mod K)/K,
+ *               w_l = 1 - w_h (both rounded to float32)    balancer.py:274-282
  *   geometry    az, sin_az, sr_surface, wave_dinc            sar_raw_nompi.py:352-369
  *   Bragg scene sum_pm amp * exp(i phase_pm) * s_pm          sar_raw_nompi.py:470-480
  *   beam        sinc_bp(L_tx) * sinc_bp(L_rx)                sar_raw_nompi.py:485-486
@@ -42,11 +43,13 @@ static double np_sinc(double x) {
   return sin(y) / y;
 }
 
-static double lerp(const float* f, size_t o0, size_t plane, int rk, double w) {
-  double a = (double)f[o0];
-  if (rk == 0) return a;
-  double b = (double)f[o0 + plane];
-  return (1. - w) * a + w * b;
+static double lerp(const float* f, size_t o0, size_t plane, int rk, float wl, float wh) {
+  const float a = f[o0];
+  if (rk == 0) return (double)a;
+  const float b = f[o0 + plane];
+  const float x = wl * a;
+  const float y = wh * b;
+  return (double)(x + y);
 }
 
 int oracle_echo(const oracle_desc* d) {
@@ -57,7 +60,8 @@ int oracle_echo(const oracle_desc* d) {
   for (int ip = 0; ip < npulse; ++ip) {
     const int p = d->p0 + ip;
     const int m = p / d->K, rk = p % d->K;
-    const double w = (double)rk / (double)d->K;
+    const float wh = (float)((double)rk / (double)d->K);
+    const float wl = (float)(1.0 - (double)wh);
     const size_t so = (size_t)(m - d->snap0) * plane;
     const int rs = d->bragg ? d->rcs_slot[p] - d->rcs0 : 0;
     const double t_now = p * d->t_step;
@@ -75,9 +79,9 @@ int oracle_echo(const oracle_desc* d) {
       }
       for (int x = 0; x < d->nx; ++x) {
         const size_t idx = (size_t)y * d->nx + x;
-        const double Dx = lerp(d->dx, so + idx, plane, rk, w);
-        const double Dy = lerp(d->dy, so + idx, plane, rk, w);
-        const double Dz = lerp(d->dz, so + idx, plane, rk, w);
+        const double Dx = lerp(d->dx, so + idx, plane, rk, wl, wh);
+        const double Dy = lerp(d->dy, so + idx, plane, rk, wl, wh);
+        const double Dz = lerp(d->dz, so + idx, plane, rk, wl, wh);
         double srs = d->sr[x] - d->cos_inc[x] * Dz;
         srs = srs + mig;
         srs = srs + Dx * d->sin_inc[x];

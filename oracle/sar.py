@@ -37,14 +37,18 @@ class SnapshotCache:
 
 
 def lerp_fields(snaps, sc, p, names):
-    """FP64 fields at pulse p: (1-w)*S[m] + w*S[m+1], w = (p mod K)/K."""
+    """Fields at pulse p between the snapshots m = p // K and m + 1, in float32
+    as the reference's snapshot balancer interpolates (balancer.py:274-282):
+    w_h = float32(w), w_l = float32(1 - w_h), w_l * S[m] + w_h * S[m+1], with
+    w = (p mod K)/K. Returned as float64 arrays (the range formula promotes them)."""
     m, r = divmod(int(p), sc.K)
     s0 = snaps.get(m)
     if r == 0:
         return {n: s0[n].astype(np.float64) for n in names}
-    w = r / sc.K
+    w_h = np.float32(r / sc.K)
+    w_l = np.float32(1. - w_h)
     s1 = snaps.get(m + 1)
-    return {n: (1. - w) * s0[n].astype(np.float64) + w * s1[n].astype(np.float64)
+    return {n: (w_l * s0[n].astype(np.float32) + w_h * s1[n].astype(np.float32)).astype(np.float64)
             for n in names}
 
 

@@ -84,3 +84,26 @@ def test_subbin_round_half_to_even():
     assert d[0] == 3 and sb[0] == 2
     d, sb = oecho.bin_index(np.array([3.625]), 1.0, 0.0)
     assert sb[0] == 8
+
+
+def test_lerp_is_balancer_float32():
+    """Between snapshots the fields follow the reference balancer's float32
+    interpolation (balancer.py:274-282): float32 weights, float32 products and
+    sum. A case where FP64 evaluation rounds differently pins the precision."""
+    import types
+    import oracle.sar as osar
+    a = np.array([[1 + 2 ** -23, 3.0]], dtype=np.float32)
+    b = np.array([[1.0, np.float32(1 / 3)]], dtype=np.float32)
+    snaps = {0: {'Dz': a}, 1: {'Dz': b}}
+    sc = types.SimpleNamespace(K=3)
+    cache = types.SimpleNamespace(get=snaps.get)
+    got = osar.lerp_fields(cache, sc, 1, ('Dz',))['Dz']
+    wh = np.float32(1 / 3)
+    wl = np.float32(1 - wh)
+    want = (np.float32(wl * a) + np.float32(wh * b)).astype(np.float64)
+    fp64 = (1 - 1 / 3) * a.astype(np.float64) + (1 / 3) * b.astype(np.float64)
+    assert got.dtype == np.float64
+    assert np.array_equal(got, want)
+    assert not np.array_equal(got, fp64)
+    # at a snapshot pulse the snapshot itself
+    assert np.array_equal(osar.lerp_fields(cache, sc, 3, ('Dz',))['Dz'], b.astype(np.float64))
