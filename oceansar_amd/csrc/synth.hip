@@ -97,6 +97,83 @@ __device__ __forceinline__ double2 herm(int m, double kx, double ky, double ik, 
   return make_double2(0.5 * (g.x + gm.x), 0.5 * (g.y - gm.y));
 }
 
+// ---- in-register inverse DFTs, X[k] = sum_r x[r] exp(+2 pi i r k / R) ----
+__device__ __forceinline__ double2 cadd2(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub2(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul_i(double2 a) { return make_double2(-a.y, a.x); }  // i a
+
+__device__ __forceinline__ void dft4(double2& x0, double2& x1, double2& x2, double2& x3) {
+  const double2 s02 = cadd2(x0, x2), d02 = csub2(x0, x2);
+  const double2 s13 = cadd2(x1, x3), d13 = csub2(x1, x3);
+  x0 = cadd2(s02, s13);
+  x1 = cadd2(d02, cmul_i(d13));
+  x2 = csub2(s02, s13);
+  x3 = csub2(d02, cmul_i(d13));
+}
+
+// exp(+2 pi i m / 16) for m = 0..9 (the products r1 k2 of the radix-16 split)
+__device__ __forceinline__ double2 w16(int m) {
+  constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173;
+  constexpr double h = 0.70710678118654752440;
+  switch (m) {
+    case 0: return make_double2(1., 0.);
+    case 1: return make_double2(c1, s1);
+    case 2: return make_double2(h, h);
+    case 3: return make_double2(s1, c1);
+    case 4: return make_double2(0., 1.);
+    case 6: return make_double2(-h, h);
+    case 9: return make_double2(-c1, -s1);
+    default: return make_double2(0., 0.);  // not used
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void dft(double2 (&x)[R]) {
+  if constexpr (R == 2) {
+    const double2 a = x[0];
+    x[0] = cadd2(a, x[1]);
+    x[1] = csub2(a, x[1]);
+  } else if constexpr (R == 4) {
+    dft4(x[0], x[1], x[2], x[3]);
+  } else if constexpr (R == 8) {
+    // r = r1 + 4 r2, k = 2 k1 + k2: radix-2 over r2, twiddle w8^(r1 k2), radix-4 over r1
+    double2 a0[4], a1[4];
+#pragma unroll
+    for (int r1 = 0; r1 < 4; ++r1) {
+      a0[r1] = cadd2(x[r1], x[r1 + 4]);
+      a1[r1] = cmul2(csub2(x[r1], x[r1 + 4]), w16(2 * r1));
+    }
+    dft4(a0[0], a0[1], a0[2], a0[3]);
+    dft4(a1[0], a1[1], a1[2], a1[3]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      x[2 * k1] = a0[k1];
+      x[2 * k1 + 1] = a1[k1];
+    }
+  } else {
+    static_assert(R == 16, "radix 2, 4, 8 or 16");
+    // r = r1 + 4 r2, k = 4 k1 + k2: radix-4 over r2, twiddle w16^(r1 k2), radix-4 over r1
+    double2 a[4][4];
+#pragma unroll
+    for (int r1 = 0; r1 < 4; ++r1) {
+      a[r1][0] = x[r1];
+      a[r1][1] = x[r1 + 4];
+      a[r1][2] = x[r1 + 8];
+      a[r1][3] = x[r1 + 12];
+      dft4(a[r1][0], a[r1][1], a[r1][2], a[r1][3]);
+#pragma unroll
+      for (int k2 = 1; k2 < 4; ++k2)
+        if (r1 > 0) a[r1][k2] = cmul2(a[r1][k2], w16(r1 * k2));
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+      dft4(a[0][k2], a[1][k2], a[2][k2], a[3][k2]);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) x[4 * k1 + k2] = a[k1][k2];
+    }
+  }
+}
+
 // Twiddle exp(+2 pi i k / N) from the two-level LDS table: tw[k & 63] for the
 // low six bits times tw[64 + (k >> 6)] for the rest (one FP64 complex product,
 // within 2 ulp of the directly rounded value).
@@ -111,71 +188,96 @@ __device__ __forceinline__ void load_twiddles(double2* dst, const double2* src) 
   for (int i = threadIdx.x; i < 128; i += blockDim.x) dst[i] = src[i];
 }
 
-// In-LDS inverse FFT of nseq power-of-two sequences (Stockham, radix 4 with a
-// final radix-2 stage when log2 N is odd). Element e of sequence s lives at
-// buf[s * ss + e * es]; tw is the two-level twiddle table in LDS. Results stay
-// in buf. SEQ_FAST: consecutive threads take consecutive sequences (nseq a
-// power of two, log2 given), for interleaved layouts (es = nseq, ss = 1) where
-// that keeps every LDS access of a wave contiguous; otherwise consecutive
-// threads take consecutive butterflies of one sequence.
-template <int NT, int BPT, bool SEQ_FAST>
+// One radix-R pass of the inverse Stockham FFT (Ns = product of the radices
+// of the passes before). Butterfly j of a sequence reads x_r = e[j + r N/R],
+// multiplies by w^(r jm N/(Ns R)) (jm = j mod Ns, w = exp(2 pi i / N)), takes
+// the radix-R DFT and writes y_r to e[(j - jm) R + jm + r Ns]. Every thread
+// loads all of its butterflies before the barrier, so the pass runs in place;
+// at most BPT butterflies per thread (checked by the caller's sizes).
+template <int NT, int R, int BPT, bool SEQ_FAST>
+__device__ __forceinline__ void stockham_pass(double2* buf, int logN, int Ns, int logNs,
+                                              int nseq, int lseq, int ss, int es,
+                                              const double2* tw) {
+  constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
+  const int tid = threadIdx.x;
+  const int lnb = logN - LR;
+  const int nb = 1 << lnb;
+  const int total = nseq << lnb;
+  double2 v[BPT][R];
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int id = tid + b * NT;
+    if (id < total) {
+      const int s = SEQ_FAST ? (id & (nseq - 1)) : (id >> lnb);
+      const int j = SEQ_FAST ? (id >> lseq) : (id & (nb - 1));
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = buf[s * ss + (j + r * nb) * es];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int id = tid + b * NT;
+    if (id < total) {
+      const int s = SEQ_FAST ? (id & (nseq - 1)) : (id >> lnb);
+      const int j = SEQ_FAST ? (id >> lseq) : (id & (nb - 1));
+      const int jm = j & (Ns - 1);
+      const int step = jm << (logN - logNs - LR);   // jm * N / (Ns * R)
+      // w^(r step) as a balanced product tree of w^step (depth <= 2 log2 R)
+      double2 wr[R];
+      wr[0] = make_double2(1., 0.);
+      if (R > 1) wr[1] = twiddle(tw, step);
+#pragma unroll
+      for (int r = 2; r < R; ++r) {
+        const int hi = (r & (r - 1)) == 0 ? r / 2 : (1 << (31 - __builtin_clz(r)));
+        wr[r] = cmul2(wr[hi], wr[r - hi]);
+      }
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[b][r] = cmul2(v[b][r], wr[r]);
+      dft<R>(v[b]);
+      const int o = (j - jm) * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[s * ss + (o + r * Ns) * es] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+// In-LDS inverse FFT of nseq power-of-two sequences, element e of sequence s
+// at buf[s * ss + e * es]; tw is the two-level twiddle table in LDS; results
+// stay in buf. Passes take radix 2^min(left, LRMAX) (one step smaller when that
+// would leave a radix-2 pass). EMAX bounds nseq * N / NT, the elements one
+// thread holds in a pass. SEQ_FAST: consecutive threads take consecutive
+// sequences (nseq a power of two, log2 lseq), for interleaved layouts
+// (es = nseq, ss = 1) where that keeps every LDS access of a wave contiguous;
+// otherwise consecutive threads take consecutive butterflies of one sequence.
+template <int NT, int EMAX, int LRMAX, bool SEQ_FAST>
 __device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int lseq, int ss, int es,
                          const double2* tw) {
-  const int tid = threadIdx.x;
-  int Ns = 1, logNs = 0;
-  int left = logN;
+  int Ns = 1, logNs = 0, left = logN;
   while (left > 0) {
-    const int lr = left >= 2 ? 2 : 1;  // log2 of the radix
-    const int R = 1 << lr;
-    const int lnb = logN - lr;          // butterflies per sequence = N / R = 1 << lnb
-    const int nb = 1 << lnb;
-    const int total = nseq << lnb;
-    double2 v[BPT][4];
-#pragma unroll
-    for (int b = 0; b < BPT; ++b) {
-      const int id = tid + b * NT;
-      if (id < total) {
-        const int s = SEQ_FAST ? (id & (nseq - 1)) : (id >> lnb);
-        const int j = SEQ_FAST ? (id >> lseq) : (id & (nb - 1));
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (r < R) v[b][r] = buf[s * ss + (j + r * nb) * es];
-      }
+    int lr = left < LRMAX ? left : LRMAX;
+    if (left - lr == 1 && lr > 2) --lr;
+    switch (lr) {
+      case 1:
+        stockham_pass<NT, 2, (EMAX / 2 > 0 ? EMAX / 2 : 1), SEQ_FAST>(buf, logN, Ns, logNs, nseq,
+                                                                      lseq, ss, es, tw);
+        break;
+      case 2:
+        stockham_pass<NT, 4, (EMAX / 4 > 0 ? EMAX / 4 : 1), SEQ_FAST>(buf, logN, Ns, logNs, nseq,
+                                                                      lseq, ss, es, tw);
+        break;
+      case 3:
+        if constexpr (LRMAX >= 3)
+          stockham_pass<NT, 8, (EMAX / 8 > 0 ? EMAX / 8 : 1), SEQ_FAST>(buf, logN, Ns, logNs,
+                                                                        nseq, lseq, ss, es, tw);
+        break;
+      default:
+        if constexpr (LRMAX >= 4)
+          stockham_pass<NT, 16, (EMAX / 16 > 0 ? EMAX / 16 : 1), SEQ_FAST>(buf, logN, Ns, logNs,
+                                                                           nseq, lseq, ss, es, tw);
+        break;
     }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < BPT; ++b) {
-      const int id = tid + b * NT;
-      if (id < total) {
-        const int s = SEQ_FAST ? (id & (nseq - 1)) : (id >> lnb);
-        const int j = SEQ_FAST ? (id >> lseq) : (id & (nb - 1));
-        const int jm = j & (Ns - 1);
-        const int step = jm << (logN - logNs - lr);   // jm * N / (Ns * R)
-        if (R == 4) {
-          const double2 x0 = v[b][0];
-          const double2 x1 = cmul2(v[b][1], twiddle(tw, step));
-          const double2 x2 = cmul2(v[b][2], twiddle(tw, 2 * step));
-          const double2 x3 = cmul2(v[b][3], twiddle(tw, 3 * step));
-          const double2 s02 = make_double2(x0.x + x2.x, x0.y + x2.y);
-          const double2 d02 = make_double2(x0.x - x2.x, x0.y - x2.y);
-          const double2 s13 = make_double2(x1.x + x3.x, x1.y + x3.y);
-          const double2 d13 = make_double2(x1.x - x3.x, x1.y - x3.y);
-          // inverse: X1 = d02 + i d13, X3 = d02 - i d13
-          const int o = (j - jm) * 4 + jm;
-          buf[s * ss + o * es] = make_double2(s02.x + s13.x, s02.y + s13.y);
-          buf[s * ss + (o + Ns) * es] = make_double2(d02.x - d13.y, d02.y + d13.x);
-          buf[s * ss + (o + 2 * Ns) * es] = make_double2(s02.x - s13.x, s02.y - s13.y);
-          buf[s * ss + (o + 3 * Ns) * es] = make_double2(d02.x + d13.y, d02.y - d13.x);
-        } else {
-          const double2 x0 = v[b][0];
-          const double2 x1 = cmul2(v[b][1], twiddle(tw, step));
-          const int o = (j - jm) * 2 + jm;
-          buf[s * ss + o * es] = make_double2(x0.x + x1.x, x0.y + x1.y);
-          buf[s * ss + (o + Ns) * es] = make_double2(x0.x - x1.x, x0.y - x1.y);
-        }
-      }
-    }
-    __syncthreads();
     Ns <<= lr;
     logNs += lr;
     left -= lr;
@@ -184,6 +286,7 @@ __device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int lseq, int 
 
 constexpr int kRowThreads = 512;
 constexpr int kColThreads = 512;
+constexpr int kColElems = 8192;                  // cb * ny: elements of one column block
 constexpr int kRowPairs = kMaxN / kRowThreads;   // column pairs a row thread owns
 
 // Pass 1: one workgroup per row pair (j, -j) and snapshot. Thread t owns the
@@ -254,7 +357,7 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
       S[(nrow - 1) * nx + im] = make_double2(hc.x - hd.y, hc.y + hd.x);
     }
     __syncthreads();
-    lds_ifft<kRowThreads, 4, false>(S, nx, a.logx, nrow, 0, nx, 1, tw);
+    lds_ifft<kRowThreads, 2 * kMaxN / kRowThreads, 3, false>(S, nx, a.logx, nrow, 0, nx, 1, tw);
     double2* w = a.work + (size_t)(ti * a.nq + qi) * a.ny * nx;
     for (int e = tid; e < nrow * nx; e += kRowThreads) {
       const int r = e >> a.logx, i = e & (nx - 1);
@@ -284,7 +387,7 @@ __global__ __launch_bounds__(kColThreads) void synth_cols_kernel(SynthArgs a) {
   const double2* w = a.work + (size_t)(ti * a.nq + qi) * ny * nx + (size_t)xb * ny * cb;
   for (int e = tid; e < ny * cb; e += kColThreads) C[e] = w[e];
   __syncthreads();
-  lds_ifft<kColThreads, 8, true>(C, ny, a.logy, cb, a.lcb, 1, cb, tw);
+  lds_ifft<kColThreads, kColElems / kColThreads, 4, true>(C, ny, a.logy, cb, a.lcb, 1, cb, tw);
   const QPlan q = a.q[qi];
   const double scale = 1.0 / ((double)nx * (double)ny);
   const double t = a.times[ti];
@@ -434,7 +537,7 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   if (rc) return rc;
   rc = twiddles(ctx, d->ny, &a.twy);
   if (rc) return rc;
-  int cb = (8192) / d->ny;
+  int cb = kColElems / d->ny;
   if (cb > 8) cb = 8;
   if (cb > d->nx) cb = d->nx;
   a.cb = cb;
