@@ -91,7 +91,7 @@ def main():
     ap.add_argument('--workload', default='large', choices=configs_names())
     ap.add_argument('--pulses', type=int, default=None, help='shorter scene (testing only)')
     ap.add_argument('--chunk', type=int, default=32, help='snapshot intervals per launch group')
-    ap.add_argument('--cpu-sample', type=int, default=32)
+    ap.add_argument('--cpu-sample', type=int, default=96)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--copy-probe-mib', type=int, default=0,
                     help='after timing, run one device copy of this many MiB (a known-size '
