@@ -153,7 +153,8 @@ typedef struct osar_bragg_desc {
   int32_t pol;              /* 0 = vv, 1 = hh, 2 = DP (hh then vv) */
   int32_t spread;           /* 0 = elfouhaily, 1 = cos2s */
   int32_t n_spec;           /* spectrum table length */
-  int32_t pulse;            /* refresh pulse (sets the azimuth of every row) */
+  int32_t pulse;            /* refresh pulse (sets the azimuth of every row) when
+                               n_refresh == 0 */
   double k0, d, wind_dir, wind_U, wind_fetch;
   double dx, dy;            /* facet size (RCS = NRCS * dx * dy) */
   double t_step, t_span, v_ground, gr0;
@@ -170,9 +171,20 @@ typedef struct osar_bragg_desc {
   const double* spec_S;     /* [n_spec] */
   const float* diffx;       /* [nx][ny] (x-major) */
   const float* diffy;
-  float* amp;               /* out: [n_pol][2][nx][ny] sqrt(RCS) */
-  double* rcs_sum;          /* out: [n_pol] sum of RCS over facets and components
-                               (accumulated; zero it first) */
+  float* amp;               /* out: [n_pol][2][nx][ny] sqrt(RCS) per refresh */
+  double* rcs_sum;          /* out: [n_refresh][n_pol] sum of RCS over facets and
+                               components (accumulated; zero it first); may be NULL */
+  /* Several refreshes in one launch (n_refresh > 0): refresh i uses pulse
+     refresh_pulse[i] and the slopes at diffx/diffy + (refresh_snap[i] - snap0) *
+     diff_stride,
+     and writes amp + i * amp_stride and rcs_sum + i * n_pol. n_refresh == 0 is
+     one refresh at `pulse` with the slopes at diffx/diffy. */
+  int32_t n_refresh;
+  int32_t snap0;            /* snapshot index held in slot 0 of diffx/diffy */
+  const int32_t* refresh_pulse;  /* device [n_refresh] */
+  const int32_t* refresh_snap;   /* device [n_refresh] */
+  int64_t diff_stride;      /* elements between snapshot slots of diffx/diffy */
+  int64_t amp_stride;       /* elements between amplitude slots */
 } osar_bragg_desc;
 
 int osar_facet_bragg(osar_ctx* ctx, const osar_bragg_desc* d, void* stream);

@@ -34,7 +34,6 @@ struct BraggConst {
   double cK_p, cK_m, a_0, a_m;   // Delta(k) = tanh(a_0 + 4 (cK/cK_p)^2.5 + a_m (cK_m/cK)^2.5)
   double cos_wd, sin_wd;         // wind direction
   double sq_re, sq_im;           // sqrt(epsilon_sw), principal branch
-  double az_now;                 // (t_now - t_span / 2) v_ground
   double dxdy;                   // float32(dx) * float32(dy) as the reference multiplies
 };
 
@@ -57,7 +56,6 @@ BraggConst bragg_const(const osar_bragg_desc& d) {
   const double eps_abs = std::hypot(73., 18.);
   c.sq_re = std::sqrt((eps_abs + 73.) / 2.);
   c.sq_im = 18. / (2. * c.sq_re);
-  c.az_now = (d.pulse * d.t_step - d.t_span / 2.) * d.v_ground;
   c.dxdy = (double)(float)d.dx * (double)(float)d.dy;
   return c;
 }
@@ -128,14 +126,21 @@ constexpr int kThreads = 256;
 // sums, and the wrapped spreading angle th of the two Bragg components by
 // cos(th) (th and th + pi have opposite cosines). The half-plane test
 // |th| < pi/2 becomes cos(th) > 0 (cos(th) < 0 for the cos^2 model's zero).
-// Grid-stride over facets; the per-thread RCS sums are reduced per workgroup
-// and added to one of kSlots partial sums (spreading the atomics over
-// addresses), which bragg_fold_kernel folds into rcs_sum.
+// Grid (workgroups per refresh, refreshes): refresh i = blockIdx.y reads the
+// slopes of snapshot slot refresh_snap[i], writes amplitude slot i and adds
+// its RCS sums to rcs_sum[i][pol] (one atomic per workgroup and pol). Each
+// workgroup grid-strides over the facets of its refresh.
 template <bool UNIFORM>
-__global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d, BraggConst c,
-                                                               double* slots) {
+__global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d, BraggConst c) {
   const size_t n = (size_t)d.nx * d.ny;
   const int npol = d.pol == 2 ? 2 : 1;
+  const int ri = blockIdx.y;
+  const int pulse = d.n_refresh > 0 ? d.refresh_pulse[ri] : d.pulse;
+  const size_t snap = d.n_refresh > 0 ? (size_t)(d.refresh_snap[ri] - d.snap0) : 0;
+  const float* diffx_s = d.diffx + snap * d.diff_stride;
+  const float* diffy_s = d.diffy + snap * d.diff_stride;
+  float* amp_s = d.amp + (size_t)ri * d.amp_stride;
+  const double az_now = (pulse * d.t_step - d.t_span / 2.) * d.v_ground;
   double sums[2] = {0., 0.};
   for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n;
        idx += (size_t)gridDim.x * kThreads) {
@@ -143,12 +148,12 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
     const int col = (int)(idx / d.ny);
     const int row = (int)(idx - (size_t)col * d.ny);
     // azimuth angle az = asin(arg) of the row (sar_raw_nompi.py:363)
-    const double arg = (d.y_row[row] - c.az_now) / d.gr0;
+    const double arg = (d.y_row[row] - az_now) / d.gr0;
     const double sin_az = fabs(arg) <= 1. ? arg : NAN;
     const double cos_az = sqrt(1. - arg * arg);
     const double cos_inc = d.cos_inc[col], sin_inc = d.sin_inc[col];
-    const double diffx = (double)d.diffx[idx];
-    const double diffy = (double)d.diffy[idx];
+    const double diffx = (double)diffx_s[idx];
+    const double diffy = (double)diffy_s[idx];
     // parallel and orthogonal slopes s_p = atan(u), s_t = atan(v)
     double cos_sp, sin_sp, cos_st, sin_st;
     cos_sin_atan(cos_az * diffx + sin_az * diffy, &cos_sp, &sin_sp);
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
         // romeiser97.py:366-392 (DP: hh masks only the plus component)
         const bool mask = bad && !(d.pol == 2 && is_hh && comp == 0);
         if (mask) rcs = 0.;
-        d.amp[((size_t)ip * 2 + comp) * n + idx] = (float)sqrt(rcs);
+        amp_s[((size_t)ip * 2 + comp) * n + idx] = (float)sqrt(rcs);
         sums[ip] += rcs;
       }
     }
@@ -220,26 +225,13 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
     red[wave][1] = sums[1];
   }
   __syncthreads();
-  if (threadIdx.x == 0 && slots) {
+  if (threadIdx.x == 0 && d.rcs_sum) {
     for (int ip = 0; ip < npol; ++ip) {
       double v = 0.;
       for (int w = 0; w < kThreads / 64; ++w) v += red[w][ip];
-      atomicAdd(&slots[(blockIdx.x % kBraggSlots) * 2 + ip], v);
+      atomicAdd(&d.rcs_sum[(size_t)ri * npol + ip], v);
     }
   }
-}
-
-// Adds the partial sums to rcs_sum and clears them for the next call (a
-// kernel, not a memset, so that a captured graph replays it correctly).
-__global__ void bragg_fold_kernel(double* slots, double* rcs_sum, int npol) {
-  const int t = threadIdx.x;  // one wave, kBraggSlots == 64
-  for (int ip = 0; ip < npol; ++ip) {
-    double v = slots[t * 2 + ip];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if (t == 0) rcs_sum[ip] += v;
-  }
-  slots[t * 2] = 0.;
-  slots[t * 2 + 1] = 0.;
 }
 
 }  // namespace
@@ -263,23 +255,31 @@ extern "C" int osar_facet_bragg(osar_ctx* ctx, const osar_bragg_desc* d, void* s
   OSAR_HIP(hipSetDevice(ctx->device));
   OSAR_REQUIRE(!d->spec_uniform || d->spec_dk > 0., OSAR_EINVAL,
                "spec_uniform needs spec_dk > 0");
-  const size_t n = (size_t)d->nx * d->ny;
-  size_t want = (n + kThreads - 1) / kThreads;
-  const size_t cap = (size_t)ctx->n_cu * 8;   // grid-stride beyond 8 workgroups per CU
-  const unsigned blocks = (unsigned)(want < cap ? want : cap);
-  const BraggConst c = bragg_const(*d);
-  double* slots = d->rcs_sum ? ctx->bragg_slots : nullptr;
-  if (d->spec_uniform)
-    hipLaunchKernelGGL(facet_bragg_kernel<true>, dim3(blocks), dim3(kThreads), 0,
-                       (hipStream_t)stream, *d, c, slots);
-  else
-    hipLaunchKernelGGL(facet_bragg_kernel<false>, dim3(blocks), dim3(kThreads), 0,
-                       (hipStream_t)stream, *d, c, slots);
-  OSAR_LAUNCHED("facet_bragg_kernel");
-  if (slots) {
-    hipLaunchKernelGGL(bragg_fold_kernel, dim3(1), dim3(kBraggSlots), 0, (hipStream_t)stream,
-                       slots, d->rcs_sum, d->pol == 2 ? 2 : 1);
-    OSAR_LAUNCHED("bragg_fold_kernel");
+  const int n_ref = d->n_refresh > 0 ? d->n_refresh : 1;
+  OSAR_REQUIRE(n_ref <= 65535, OSAR_EINVAL, "n_refresh %d exceeds 65535", n_ref);
+  if (d->n_refresh > 0) {
+    OSAR_PTR(d->refresh_pulse);
+    OSAR_PTR(d->refresh_snap);
+    OSAR_REQUIRE(d->diff_stride >= (int64_t)d->nx * d->ny || d->n_refresh == 1, OSAR_EINVAL,
+                 "diff_stride smaller than a field");
+    OSAR_REQUIRE(d->amp_stride >= 2 * (int64_t)d->nx * d->ny * (d->pol == 2 ? 2 : 1) ||
+                     d->n_refresh == 1,
+                 OSAR_EINVAL, "amp_stride smaller than one refresh's amplitudes");
   }
+  const size_t n = (size_t)d->nx * d->ny;
+  // about 8 workgroups per CU in total, at least 16 per refresh, none idle
+  size_t per = ((size_t)ctx->n_cu * 8 + n_ref - 1) / n_ref;
+  if (per < 16) per = 16;
+  const size_t want = (n + kThreads - 1) / kThreads;
+  if (per > want) per = want;
+  const BraggConst c = bragg_const(*d);
+  const dim3 grid((unsigned)per, (unsigned)n_ref);
+  if (d->spec_uniform)
+    hipLaunchKernelGGL(facet_bragg_kernel<true>, grid, dim3(kThreads), 0, (hipStream_t)stream,
+                       *d, c);
+  else
+    hipLaunchKernelGGL(facet_bragg_kernel<false>, grid, dim3(kThreads), 0, (hipStream_t)stream,
+                       *d, c);
+  OSAR_LAUNCHED("facet_bragg_kernel");
   return OSAR_OK;
 }

@@ -14,10 +14,7 @@ struct osar_ctx {
   size_t max_lds;             // bytes of LDS one workgroup may allocate
   void* work;                 // synthesis workspace (grows on demand)
   size_t work_bytes;
-  double* bragg_slots;        // [kBraggSlots][2] partial RCS sums, zero between calls
 };
-
-constexpr int kBraggSlots = 64;
 
 // Thread-local last error message (osar_last_error).
 void osar_set_error(const char* fmt, ...);

@@ -43,13 +43,6 @@ extern "C" int osar_ctx_create(int device, osar_ctx** out) {
   if (prop.maxSharedMemoryPerMultiProcessor > c->max_lds)
     c->max_lds = prop.maxSharedMemoryPerMultiProcessor;
   if (c->max_lds > 160 * 1024) c->max_lds = 160 * 1024;
-  if (hipMalloc(&c->bragg_slots, sizeof(double) * 2 * kBraggSlots) != hipSuccess ||
-      hipMemset(c->bragg_slots, 0, sizeof(double) * 2 * kBraggSlots) != hipSuccess) {
-    if (c->bragg_slots) (void)hipFree(c->bragg_slots);
-    free(c);
-    osar_set_error("device allocation of the context failed");
-    return OSAR_ENOMEM;
-  }
   *out = c;
   return OSAR_OK;
 }
@@ -58,7 +51,6 @@ extern "C" int osar_ctx_destroy(osar_ctx* ctx) {
   if (!ctx) return OSAR_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->work) (void)hipFree(ctx->work);
-  if (ctx->bragg_slots) (void)hipFree(ctx->bragg_slots);
   free(ctx);
   return OSAR_OK;
 }

@@ -26,14 +26,20 @@ def synth_fields(ctx, surf, kx, ky, coefs, times, out, fields=None):
 
     kx, ky, coefs: device tensors (float32 [nx], [ny], complex64 [ny][nx]).
     out[name]: float32 device tensor [>= n_times][nx][ny] (x-major) per field.
-    Returns the device tensor of times, which must stay alive until the
-    stream has run the launch.
+    `times` is a sequence of floats (copied to the device) or a contiguous float64
+    device tensor (used in place, no host round trip). Returns the device tensor
+    of times, which must stay alive until the stream has run the launch.
     """
     import torch
     fields = fields or list(out.keys())
     if getattr(surf, 'swell_enable', False):
         raise NotImplementedError('monochromatic swell is not supported on the GPU path')
-    times_t = torch.as_tensor(np.asarray(times, dtype=np.float64), device=coefs.device)
+    if torch.is_tensor(times):
+        if times.dtype != torch.float64 or not times.is_contiguous() or times.device != coefs.device:
+            raise TypeError('times must be a contiguous float64 tensor on the surface device')
+        times_t = times
+    else:
+        times_t = torch.as_tensor(np.asarray(times, dtype=np.float64), device=coefs.device)
     d = _lib.SurfaceDesc()
     d.nx, d.ny = surf.Nx, surf.Ny
     mask = 0
@@ -74,6 +80,10 @@ def arange_grid(k):
     return float(k[0]), 0., 0
 
 
+class RunState:
+    """Buffers and position of one DeviceScene run (DeviceScene.start/step/finish)."""
+
+
 class LaunchShape:
     """Echo-accumulator launch shape: pulses per workgroup, rows per workgroup, LDS window."""
 
@@ -112,6 +122,10 @@ class DeviceScene:
             self.ind0_m = _t(np.mod(sc.scat_m.ind0, sc.scat_m.N).T, i32, dev)
             self.ind0_p = _t(np.mod(sc.scat_p.ind0, sc.scat_p.N).T, i32, dev)
             self.rcs_slot = _t(sc.rcs_slot, i32, dev)
+            if np.any(sc.rcs_refresh % sc.K):
+                raise ValueError('Bragg refresh pulses must be snapshot pulses (multiples of K)')
+            self.refresh_pulse = _t(sc.rcs_refresh, i32, dev)
+            self.refresh_snap = _t(sc.rcs_refresh // sc.K, i32, dev)
             self.spec_k = _t(sc.bragg.spec_k, f64, dev)
             self.spec_grid = arange_grid(sc.bragg.spec_k)
             self.spec_S = _t(sc.bragg.spec_S, f64, dev)
@@ -119,6 +133,8 @@ class DeviceScene:
         self.coefs = _t(surf.wave_coefs, c64, dev)
         self.kx = _t(surf.kx_s, f32, dev)
         self.ky = _t(surf.ky_s, f32, dev)
+        n_snap = sc.az_steps // sc.K + 2
+        self.snap_times = _t([sc.snapshot_time(m) for m in range(n_snap)], f64, dev)
         self.resid_max = self._resid_bound()
         self.shape = self.default_shape(lds_budget)
 
@@ -185,13 +201,37 @@ class DeviceScene:
 
     def bragg(self, diffx, diffy, pulse, amp_out, rcs_sum_out=None):
         """sqrt(RCS) of every facet at refresh pulse `pulse` into amp_out [npol][2][Ny][Nx]."""
+        d = self._bragg_desc()
+        d.pulse = int(pulse)
+        d.diffx, d.diffy = _lib.ptr(diffx), _lib.ptr(diffy)
+        d.amp = _lib.ptr(amp_out)
+        d.rcs_sum = _lib.ptr(rcs_sum_out)
+        _lib.check(self.ctx.lib.osar_facet_bragg(self.ctx.h, ctypes.byref(d), _lib.stream_ptr()))
+
+    def bragg_refreshes(self, r_first, r_last, diffx, diffy, snap0, amp_out, rcs_sum_out):
+        """Refreshes r_first..r_last in one launch: slopes from snapshot slots
+        (diffx/diffy [slot][Nx][Ny], slot 0 = snapshot snap0), amplitudes into
+        amp_out[0 .. r_last - r_first], sums into rcs_sum_out[r_first .. r_last]."""
+        n = r_last - r_first + 1
+        d = self._bragg_desc()
+        d.n_refresh = n
+        d.snap0 = int(snap0)
+        d.refresh_pulse = self.refresh_pulse.data_ptr() + 4 * r_first
+        d.refresh_snap = self.refresh_snap.data_ptr() + 4 * r_first
+        d.diffx, d.diffy = _lib.ptr(diffx), _lib.ptr(diffy)
+        d.diff_stride = diffx[0].numel()
+        d.amp = _lib.ptr(amp_out)
+        d.amp_stride = amp_out[0].numel()
+        d.rcs_sum = rcs_sum_out[r_first].data_ptr()
+        _lib.check(self.ctx.lib.osar_facet_bragg(self.ctx.h, ctypes.byref(d), _lib.stream_ptr()))
+
+    def _bragg_desc(self):
         sc, b = self.sc, self.sc.bragg
         d = _lib.BraggDesc()
         d.nx, d.ny = sc.Nx, sc.Ny
         d.pol = POL_CODE[sc.pol] if sc.pol in POL_CODE else 0
         d.spread = b.spread_id
         d.n_spec = len(b.spec_k)
-        d.pulse = int(pulse)
         d.k0, d.d, d.wind_dir, d.wind_U, d.wind_fetch = b.k0, b.d, b.wind_dir, b.wind_U, b.wind_fetch
         d.dx, d.dy = float(sc.surface.dx), float(sc.surface.dy)
         d.t_step, d.t_span, d.v_ground, d.gr0 = sc.t_step, sc.t_span, sc.v_ground, sc.gr0
@@ -201,10 +241,7 @@ class DeviceScene:
         d.spec_k0, d.spec_dk, d.spec_uniform = self.spec_grid
         k = b.spec_k
         d.spec_inv_dk = float((len(k) - 1) / (k[-1] - k[0])) if len(k) > 1 else 0.
-        d.diffx, d.diffy = _lib.ptr(diffx), _lib.ptr(diffy)
-        d.amp = _lib.ptr(amp_out)
-        d.rcs_sum = _lib.ptr(rcs_sum_out)
-        _lib.check(self.ctx.lib.osar_facet_bragg(self.ctx.h, ctypes.byref(d), _lib.stream_ptr()))
+        return d
 
     def echo_desc(self, dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape=None,
                   spec=None, spec_energy=None):
@@ -295,83 +332,104 @@ class DeviceScene:
 
         Snapshots are synthesised in chunks of `chunk_snaps` intervals (the
         boundary snapshot of a chunk is recomputed by the next one), Bragg
-        amplitudes at every refresh pulse of the chunk, then one accumulator
-        launch covers the chunk. `timer`, if given, is called as
+        amplitudes of every refresh of the chunk in one launch, then one
+        accumulator launch covers the chunk. `timer`, if given, is called as
         timer(stage, start_event, end_event) for every launch group.
         """
+        st = self.start(p0, p1, chunk_snaps, raw, shape)
+        while self.step(st, timer):
+            pass
+        return self.finish(st)
+
+    def start(self, p0=0, p1=None, chunk_snaps=32, raw=None, shape=None):
+        """Allocate the buffers of a run over pulses [p0, p1) (no launches)."""
         import torch
         sc = self.sc
-        p1 = self.n_pulses if p1 is None else p1
-        K = sc.K
-        raw = self.new_raw() if raw is None else raw
-        names = self.fields_needed()
+        st = RunState()
+        st.p0 = st.p = int(p0)
+        st.p1 = self.n_pulses if p1 is None else int(p1)
+        st.C = max(1, int(chunk_snaps))
+        st.shape = shape
+        st.raw = self.new_raw() if raw is None else raw
+        st.names = self.fields_needed()
         nx, ny = sc.Nx, sc.Ny
-        C = max(1, int(chunk_snaps))
-        buf = {n: torch.empty((C + 1, nx, ny), dtype=torch.float32, device=self.dev)
-               for n in names}
+        st.buf = {n: torch.empty((st.C + 1, nx, ny), dtype=torch.float32, device=self.dev)
+                  for n in st.names}
         nref = len(sc.rcs_refresh) if sc.bragg_enable else 0
-        rcs_sum = torch.zeros(max(nref, 1), self.npol, dtype=torch.float64, device=self.dev)
-        amp = None
+        st.rcs_sum = torch.zeros(max(nref, 1), self.npol, dtype=torch.float64, device=self.dev)
+        st.amp = None
         if sc.bragg_enable:
-            amp = torch.empty((C + 2, self.npol, 2, nx, ny), dtype=torch.float32, device=self.dev)
-        held = -1           # refresh index whose amplitudes sit in amp[0]
-        spec_energy = None
+            st.amp = torch.empty((st.C + 2, self.npol, 2, nx, ny), dtype=torch.float32,
+                                 device=self.dev)
+        st.held = -1          # refresh index whose amplitudes sit in amp[0]
+        st.spec_energy = None
         if sc.spec_enable:
-            spec_energy = torch.zeros(self.n_pulses, dtype=torch.float64, device=self.dev)
-        p = p0
-        while p < p1:
-            m0 = p // K
-            m1 = min(m0 + C, (p1 - 1) // K + 1)       # last snapshot needed
-            q1 = min(m1 * K, p1) if m1 > m0 else p1
-            if q1 <= p:
-                q1 = min(p + K, p1)
-            last = q1 - 1
-            m_last = last // K + (1 if last % K else 0)
-            times = [sc.snapshot_time(m) for m in range(m0, m_last + 1)]
+            st.spec_energy = torch.zeros(self.n_pulses, dtype=torch.float64, device=self.dev)
+        return st
+
+    def step(self, st, timer=None):
+        """Enqueue the next chunk of `st` on the current stream; False when done."""
+        if st.p >= st.p1:
+            return False
+        import torch
+        sc = self.sc
+        K = sc.K
+        p, p1, C, buf = st.p, st.p1, st.C, st.buf
+        m0 = p // K
+        m1 = min(m0 + C, (p1 - 1) // K + 1)       # last snapshot needed
+        q1 = min(m1 * K, p1) if m1 > m0 else p1
+        if q1 <= p:
+            q1 = min(p + K, p1)
+        last = q1 - 1
+        m_last = last // K + (1 if last % K else 0)
+        ev = self._events(timer)
+        self.synth(self.snap_times[m0:m_last + 1], buf, st.names)
+        self._mark(timer, 'synth', ev)
+        r0 = r1 = 0
+        if sc.bragg_enable:
+            r0 = int(sc.rcs_slot[p])
+            r1 = int(sc.rcs_slot[last])
             ev = self._events(timer)
-            self.synth(times, buf, names)
-            self._mark(timer, 'synth', ev)
-            r0 = r1 = 0
-            if sc.bragg_enable:
-                r0 = int(sc.rcs_slot[p])
-                r1 = int(sc.rcs_slot[last])
-                ev = self._events(timer)
-                for r in range(r0, r1 + 1):
-                    if r == held:
-                        continue            # computed by the previous chunk, kept in amp[0]
-                    pr = int(sc.rcs_refresh[r])
-                    mi = pr // K - m0
-                    if mi < 0:
-                        # refresh pulse before this chunk: synthesise its slopes
-                        tmp = {n: torch.empty((1, nx, ny), dtype=torch.float32, device=self.dev)
-                               for n in ('Diffx', 'Diffy')}
-                        self.synth([sc.snapshot_time(pr // K)], tmp, ['Diffx', 'Diffy'])
-                        dxs, dys = tmp['Diffx'][0], tmp['Diffy'][0]
-                    else:
-                        dxs, dys = buf['Diffx'][mi], buf['Diffy'][mi]
-                    self.bragg(dxs, dys, pr, amp[r - r0], rcs_sum[r])
-                self._mark(timer, 'bragg', ev)
-            ev = self._events(timer)
-            self.accumulate(buf['Dx'], buf['Dy'], buf['Dz'], m0, amp, r0, p, q1 - p, raw,
-                            shape, spec=buf if sc.spec_enable else None,
-                            spec_energy=spec_energy)
-            self._mark(timer, 'echo', ev)
-            if sc.bragg_enable:
-                # the last refresh in force stays in amp[0] for the next chunk
-                if r1 > r0:
-                    amp[0].copy_(amp[r1 - r0])
-                held = r1
-            p = q1
+            r_first = r0 + 1 if r0 == st.held else r0
+            if r_first == r0 and int(sc.rcs_refresh[r0]) // K < m0:
+                # refresh pulse before this chunk: synthesise its slopes alone
+                pr = int(sc.rcs_refresh[r0])
+                tmp = {n: torch.empty((1, sc.Nx, sc.Ny), dtype=torch.float32, device=self.dev)
+                       for n in ('Diffx', 'Diffy')}
+                self.synth(self.snap_times[pr // K:pr // K + 1], tmp, ['Diffx', 'Diffy'])
+                self.bragg(tmp['Diffx'][0], tmp['Diffy'][0], pr, st.amp[0], st.rcs_sum[r0])
+                r_first += 1
+            if r_first <= r1:
+                self.bragg_refreshes(r_first, r1, buf['Diffx'], buf['Diffy'], m0,
+                                     st.amp[r_first - r0:], st.rcs_sum)
+            self._mark(timer, 'bragg', ev)
+        ev = self._events(timer)
+        self.accumulate(buf['Dx'], buf['Dy'], buf['Dz'], m0, st.amp, r0, p, q1 - p, st.raw,
+                        st.shape, spec=buf if sc.spec_enable else None,
+                        spec_energy=st.spec_energy)
+        self._mark(timer, 'echo', ev)
+        if sc.bragg_enable:
+            # the last refresh in force stays in amp[0] for the next chunk
+            if r1 > r0:
+                st.amp[0].copy_(st.amp[r1 - r0])
+            st.held = r1
+        st.p = q1
+        return True
+
+    def finish(self, st):
+        """Wait for the run and return (raw device tensor, NRCS_avg numpy [pol][pulse])."""
+        sc = self.sc
+        p0, p1 = st.p0, st.p1
         nrcs = np.zeros((self.npol, self.n_pulses))
         if sc.bragg_enable:
-            sums = rcs_sum.cpu().numpy()
+            sums = st.rcs_sum.cpu().numpy()
             slots = sc.rcs_slot[p0:p1]
             nrcs[:, p0:p1] = sums[slots].T / sc.surface_area
         if sc.spec_enable:
             # specular |Esn|^2 of the last polarisation goes to every polarisation
             # (sar_raw_nompi.py:427-428)
-            nrcs[:, p0:p1] += spec_energy[p0:p1].cpu().numpy()[None, :] / sc.surface_area
-        return raw, nrcs
+            nrcs[:, p0:p1] += st.spec_energy[p0:p1].cpu().numpy()[None, :] / sc.surface_area
+        return st.raw, nrcs
 
     @staticmethod
     def _events(timer):
