@@ -83,6 +83,100 @@ def cpu_baseline(workload, sample_pulses, threads):
                                   len(set(sc.rcs_slot[p0:p1])), dt)}
 
 
+def _scenes(kws, tag):
+    from oceansar_amd import configs
+    d = tempfile.mkdtemp(prefix=tag)
+    t = time.perf_counter()
+    scenes = [configs.load_scene(configs.cfg_text(**kw), d, name='s%d.cfg' % i)
+              for i, kw in enumerate(kws)]
+    return scenes, time.perf_counter() - t
+
+
+def bench_batch(args):
+    """64 independent scenes (sides 128-1024 at 1 m, 512-4096 pulses, 1-4
+    channels, U10 3-15 m/s, any wind direction; seed 4) with every input
+    resident: the plain path one scene after another, the same scenes
+    concurrently over streams, and the captured graph replayed as one launch.
+    value = contributions / makespan of the graph replay."""
+    import torch
+    from oceansar_amd import batch
+    n = args.jobs or 64
+    scenes, t_host = _scenes(batch.mixed_workload(n=n, seed=4), 'batch')
+    tuner = batch.Autotuner()
+    t = time.perf_counter()
+    br = batch.BatchRunner(scenes, n_streams=args.streams, tuner=tuner)
+    t_setup = time.perf_counter() - t
+    W = br.contributions
+    br.run_sequential()                       # warm-up: workspaces, kernel attributes
+    seq = min(br.run_sequential() for _ in range(2))
+    conc = min(br.run_concurrent() for _ in range(2))
+    t = time.perf_counter()
+    br.capture()
+    t_capture = time.perf_counter() - t
+    graph = min(br.replay() for _ in range(3))
+    torch.cuda.synchronize()
+    return {
+        'metric': 'facet-pulse contributions/s (batching form: %d mixed scenes, one graph launch)' % n,
+        'value': W / graph, 'unit': 'contributions/s', 'n_gpus': 1, 'higher_is_better': True,
+        'dtype': 'f32 (FP64 range/phase/bins)', 'data': 'synthetic (seed 4)',
+        'config': {'workload': '%d scenes, sides 128-1024 at 1 m, 512-4096 pulses, 1-4 channels, '
+                               'U10 3-15 m/s, K=16' % n,
+                   'contributions': W, 'streams': br.n_streams,
+                   'sides': sorted(set(sc.Nx for sc in scenes)),
+                   'tuned_classes': len(tuner.cache)},
+        'makespan_s': {'plain_sequential': seq, 'concurrent_streams': conc, 'graph_replay': graph},
+        'plain_value': W / seq, 'speedup_vs_plain': seq / graph,
+        'host_s': {'scene_setup': t_host, 'upload_and_tune': t_setup, 'graph_capture': t_capture},
+    }
+
+
+def bench_serve(args):
+    """Serving form: jobs sized as the batching form plus 10 % at 2048^2 with
+    2 channels (seed 5), Poisson arrivals at 0.8 x the measured plain capacity,
+    admitted by HBM footprint under 200 GB. Reports p50/p99 latency (arrival ->
+    result on the host) and throughput, beside the same arrivals run one job at
+    a time on the plain path."""
+    import numpy as np
+    import torch
+    from oceansar_amd import batch, serve
+    n = args.jobs or 48
+    kws = batch.mixed_workload(n=n, seed=5)
+    rng = np.random.default_rng(55)
+    for kw in kws:
+        if rng.uniform() < 0.1:
+            kw.update(L=2048., dx=1., num_ch=2, b_ati=[0., 5.])
+    scenes, t_host = _scenes(kws, 'serve')
+    W = sum(sc.contributions for sc in scenes)
+    # plain capacity: the jobs back to back (every job arrived at t = 0)
+    warm = [serve.Job(i, sc, 0.) for i, sc in enumerate(scenes[:2])]
+    serve.serve_sequential(warm)
+    cap_jobs = [serve.Job(i, sc, 0.) for i, sc in enumerate(scenes)]
+    cap = serve.serve_sequential(cap_jobs)
+    capacity = W / cap['makespan_s']
+    rate = 0.8 * capacity / (W / n)                 # jobs per second
+    arrivals = serve.poisson_arrivals(n, rate, seed=5)
+    srv = serve.Server(n_slots=args.streams, hbm_budget=200e9)
+    srv.serve([serve.Job(i, sc, 0.) for i, sc in enumerate(scenes[:2])])   # warm-up
+    jobs = [serve.Job(i, sc, a) for i, (sc, a) in enumerate(zip(scenes, arrivals))]
+    got = srv.serve(jobs)
+    seq_jobs = [serve.Job(i, sc, a) for i, (sc, a) in enumerate(zip(scenes, arrivals))]
+    plain = serve.serve_sequential(seq_jobs)
+    torch.cuda.synchronize()
+    return {
+        'metric': 'p99 scene latency (serving form: %d jobs, Poisson arrivals at 0.8 x plain capacity)' % n,
+        'value': got['p99_latency_s'], 'unit': 's', 'n_gpus': 1, 'higher_is_better': False,
+        'dtype': 'f32 (FP64 range/phase/bins)', 'data': 'synthetic (seed 5)',
+        'config': {'workload': '%d jobs: sides 128-1024 at 1 m (10%% 2048^2, 2 ch), 512-4096 pulses, '
+                               '1-4 channels; HBM budget 200 GB; %d slots' % (n, srv.n_slots),
+                   'contributions': W, 'arrival_rate_per_s': rate},
+        'served': got, 'plain_sequential': plain,
+        'p50_speedup_vs_plain': plain['p50_latency_s'] / got['p50_latency_s'],
+        'p99_speedup_vs_plain': plain['p99_latency_s'] / got['p99_latency_s'],
+        'throughput_vs_plain': got['contributions_per_s'] / plain['contributions_per_s'],
+        'plain_capacity': capacity, 'host_scene_setup_s': t_host,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -93,6 +187,12 @@ def main():
     ap.add_argument('--chunk', type=int, default=32, help='snapshot intervals per launch group')
     ap.add_argument('--cpu-sample', type=int, default=96)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--form', default='plain', choices=('plain', 'batch', 'serve'),
+                    help='plain: one scene (the headline line); batch: 64 mixed scenes, '
+                         'plain vs concurrent vs one graph launch; serve: Poisson job '
+                         'arrivals, p50/p99 latency vs sequential plain execution')
+    ap.add_argument('--jobs', type=int, default=None, help='scenes (batch) or jobs (serve)')
+    ap.add_argument('--streams', type=int, default=4)
     ap.add_argument('--copy-probe-mib', type=int, default=0,
                     help='after timing, run one device copy of this many MiB (a known-size '
                          'kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters)')
@@ -101,6 +201,13 @@ def main():
         print('bench.py measures one GPU only: this build has no multi-GPU path '
               '(pulse sharding across GPUs is a design note, not implemented)', file=sys.stderr)
         sys.exit(2)
+
+    if args.form == 'batch':
+        print(json.dumps(bench_batch(args)))
+        return
+    if args.form == 'serve':
+        print(json.dumps(bench_serve(args)))
+        return
 
     import torch
     from oceansar_amd import configs

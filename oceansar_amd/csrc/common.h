@@ -16,6 +16,12 @@ struct osar_ctx {
   size_t work_bytes;
 };
 
+// Raise a kernel's dynamic LDS limit to `bytes` on the current device if it is
+// not already that high. The attribute is only set when the need grows, so
+// that steady-state launches (and launches under stream capture, after a
+// first eager run) make no other runtime calls.
+int osar_reserve_lds(const void* kernel, size_t bytes);
+
 // Thread-local last error message (osar_last_error).
 void osar_set_error(const char* fmt, ...);
 

@@ -2,6 +2,10 @@
 #include <stdarg.h>
 #include <stdlib.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "common.h"
 
 static thread_local char g_last_error[1024] = "";
@@ -16,6 +20,20 @@ void osar_set_error(const char* fmt, ...) {
 extern "C" const char* osar_last_error(void) { return g_last_error; }
 
 extern "C" int osar_version(void) { return 100; }
+
+int osar_reserve_lds(const void* kernel, size_t bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, size_t> done;
+  int dev = 0;
+  OSAR_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& have = done[{dev, kernel}];
+  if (bytes > have) {
+    OSAR_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    have = bytes;
+  }
+  return OSAR_OK;
+}
 
 extern "C" int64_t osar_struct_size(int which) {
   switch (which) {

@@ -623,8 +623,11 @@ int launch_accumulate(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipS
   OSAR_REQUIRE(lds <= ctx->max_lds, OSAR_EUNSUPPORTED,
                "echo window needs %zu B of LDS (limit %zu); lower window_bins or "
                "pulses_per_block", lds, ctx->max_lds);
-  OSAR_HIP(hipFuncSetAttribute((const void*)echo_accumulate_kernel<P, NCH, NPOL, SPEC, XTI>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  {
+    const int rc = osar_reserve_lds((const void*)echo_accumulate_kernel<P, NCH, NPOL, SPEC, XTI>,
+                                    lds);
+    if (rc) return rc;
+  }
   const int n_pgroups = (d->npulses + P - 1) / P;
   const int64_t n_items = (int64_t)n_pgroups * ((d->ny + R - 1) / R);
   OSAR_REQUIRE(n_items < INT_MAX, OSAR_EINVAL, "too many workgroups (%lld)", (long long)n_items);

@@ -545,10 +545,12 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   hipStream_t s = (hipStream_t)stream;
   const size_t lds_rows = (2 * (size_t)d->nx + 128) * sizeof(double2);
   const size_t lds_cols = ((size_t)cb * d->ny + 128) * sizeof(double2);
-  OSAR_HIP(hipFuncSetAttribute((const void*)synth_rows_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows));
-  OSAR_HIP(hipFuncSetAttribute((const void*)synth_cols_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cols));
+  {
+    const int rc1 = osar_reserve_lds((const void*)synth_rows_kernel, lds_rows);
+    if (rc1) return rc1;
+    const int rc2 = osar_reserve_lds((const void*)synth_cols_kernel, lds_cols);
+    if (rc2) return rc2;
+  }
   hipLaunchKernelGGL(synth_rows_kernel, dim3(d->ny / 2 + 1, d->n_times), dim3(kRowThreads),
                      lds_rows, s, a);
   OSAR_LAUNCHED("synth_rows_kernel");

@@ -98,10 +98,14 @@ class LaunchShape:
 
 
 class DeviceScene:
-    def __init__(self, sc, device=0, lds_budget=160 * 1024):
+    def __init__(self, sc, device=0, lds_budget=160 * 1024, ctx=None, shape=None):
+        """Upload the scene's tables. `ctx`: the library context to launch with
+        (each concurrently used stream needs its own: a context owns the
+        synthesis workspace); default the device's shared context. `shape`: a
+        LaunchShape for the accumulator instead of default_shape()."""
         import torch
         self.sc = sc
-        self.ctx = _lib.context(device)
+        self.ctx = ctx if ctx is not None else _lib.context(device)
         self.dev = torch.device('cuda', device)
         dev = self.dev
         f64, f32, i32, c64 = torch.float64, torch.float32, torch.int32, torch.complex64
@@ -136,7 +140,7 @@ class DeviceScene:
         n_snap = sc.az_steps // sc.K + 2
         self.snap_times = _t([sc.snapshot_time(m) for m in range(n_snap)], f64, dev)
         self.resid_max = self._resid_bound()
-        self.shape = self.default_shape(lds_budget)
+        self.shape = shape if shape is not None else self.default_shape(lds_budget)
 
     # ------------------------------------------------------------------
     def _extend(self, rs):
