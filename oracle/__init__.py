@@ -17,12 +17,19 @@ Contents and how each is pinned (see tests/test_oracle_*.py):
                           slopes, curvatures, choppy forms) and Parseval.
   bragg.rcs_romeiser97 a9 pinned by the flat-surface SPM closed form, slope
                           symmetry, the bad-facet mask and spread half plane.
-  ka.field_spa        a10 pinned by a paraboloid with known curvatures and by
-                          the flat-facet normal-incidence limit.
+  ka.field_spa        a10 pinned by the geometrical-optics RCS of a specular
+                          point, pi R1 R2 |R(0)|^2 (caps, rotated axes, a
+                          saddle, both polarisations), the facet-boundary cut
+                          and the exp(i q.delta) phase of the stationary point.
   echo.profile_integrator / chan_profile  a13/a14  pinned by brute-force
                           sinc evaluation, a point target, edge dropping,
                           linearity and a one-bin shift.
+  sar.lerp_fields     the balancer's float32 interpolation, pinned by a case
+                          where FP64 evaluation rounds differently.
   sar.simulate        a1/a6/a11/a12  per-pulse loop with integer cadence;
-                          pinned by superposition of point targets and by
-                          the C restatement (c/oracle_echo.c) agreeing with it.
+                          built from the pinned parts above; the C
+                          restatement (c/oracle_echo.c) agrees with it to 1e-12.
+                          Its own geometry and phase terms (range migration,
+                          Bragg phase rate, beam, channel phasors) have no
+                          separate closed-form pin.
 """
