@@ -87,6 +87,10 @@ typedef struct osar_echo_desc {
   int32_t rows_per_block;
   int32_t window_bins;      /* LDS range-line window per (pol, ch, pulse) */
   int32_t pol;              /* 0 = vv, 1 = hh, 2 = DP (line 0 hh, line 1 vv) */
+  int32_t cols_per_block;   /* columns per workgroup (0 = all); below nx, sr_col must
+                               be non-decreasing (the window starts at the chunk's
+                               first column) */
+  int32_t reserved;
   /* FP64 scalars */
   double t_step, t_span, v_ground, sr0, gr0, k0;
   double min_sr, binsize;   /* bin position x = (sr - min_sr) / binsize */

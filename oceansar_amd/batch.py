@@ -76,13 +76,20 @@ class Autotuner:
 
     @staticmethod
     def candidates(ds, lds_budget=160 * 1024):
+        """The default shape, and P in (8, 4, 2) over column chunks of the full
+        width down to 1/4 (chunks need a monotonic slant-range table)."""
         sc = ds.sc
-        out = []
+        out = [ds.default_shape(lds_budget)]
         for P in (8, 4, 2):
             R = min(8 * (64 // P), sc.Ny)
-            W = ds.window_bins(R)
-            if ds.lds_bytes(P, sc.num_ch, ds.npol, W) <= lds_budget:
-                out.append(LaunchShape(P, R, W))
+            for cols in (0, sc.Nx // 2, sc.Nx // 4):
+                if cols and (cols < 64 or not ds.sr_monotonic):
+                    continue
+                W = ds.window_bins(R, cols)
+                if ds.lds_bytes(P, sc.num_ch, ds.npol, W) <= lds_budget:
+                    s = LaunchShape(P, R, W, cols)
+                    if all(s.as_list() != o.as_list() for o in out):
+                        out.append(s)
         return out
 
     def lookup(self, sc):
@@ -116,8 +123,7 @@ class Autotuner:
                 if best is None or per_pulse < best[0]:
                     best = (per_pulse, shape, chunk)
         _, shape, chunk = best
-        self.cache[self.key(sc)] = {'shape': [shape.pulses_per_block, shape.rows_per_block,
-                                              shape.window_bins], 'chunk': chunk,
+        self.cache[self.key(sc)] = {'shape': shape.as_list(), 'chunk': chunk,
                                     'us_per_pulse': best[0] * 1e6}
         self.save()
         return shape, chunk

@@ -186,6 +186,13 @@ __device__ bool ka_spa(double k0, int pol_code, double th, double ph, double Dx,
   return true;
 }
 
+// Read-only element at a wave-uniform index through the constant address space,
+// which the compiler serves with scalar loads into scalar registers.
+__device__ __forceinline__ double uniform_load(const double* p, int i) {
+  typedef const __attribute__((address_space(4))) double cdouble;
+  return ((cdouble*)p)[__builtin_amdgcn_readfirstlane(i)];
+}
+
 // Scatterer series offsets of one facet. They are loaded one column ahead of
 // the facet's other inputs, so that the gather of the scatterer values (which
 // depends on them) never waits for a load issued in the same column.
@@ -217,12 +224,14 @@ template <int NPOL, bool SPEC, bool XTI>
 __device__ __forceinline__ void load_facet(const osar_echo_desc& d, int x, size_t idx, size_t so,
                                            size_t plane, int rk, int p, int rslot, bool bragg,
                                            const FacetIdx& fi, FacetIn<NPOL, SPEC>& f) {
-  f.sr = d.sr_col[x];
-  f.ci = d.cos_inc[x];
-  f.si = d.sin_inc[x];
-  if (bragg) f.cap = d.cap_rate[x];
-  if (SPEC) f.inc = d.inc_col[x];
-  if (XTI) f.dinc = d.dinc_col[x];
+  // the column x is the same for every lane of the wave: read the column
+  // tables through the scalar cache into scalar registers
+  f.sr = uniform_load(d.sr_col, x);
+  f.ci = uniform_load(d.cos_inc, x);
+  f.si = uniform_load(d.sin_inc, x);
+  if (bragg) f.cap = uniform_load(d.cap_rate, x);
+  if (SPEC) f.inc = uniform_load(d.inc_col, x);
+  if (XTI) f.dinc = uniform_load(d.dinc_col, x);
   f.dx0 = d.dx[so];
   f.dy0 = d.dy[so];
   f.dz0 = d.dz[so];
@@ -339,8 +348,12 @@ __device__ __forceinline__ int xcd_item(int h, int G) {
 }
 
 template <int P, int NCH, int NPOL, bool SPEC, bool XTI>
-__global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_desc d, int W,
+// Two workgroups per CU (4 waves per SIMD, at most 128 VGPRs) for the plain
+// Bragg kernels when the LDS lines fit twice (column chunks make them fit);
+// the specular and XTI variants keep their registers (2 waves per SIMD).
+__global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumulate_kernel(osar_echo_desc d, int W,
                                                                    int R, int LS, int n_pgroups,
+                                                                   int n_xchunks, int CX,
                                                                    int n_items) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   constexpr int NL = NPOL * NCH * P;
@@ -349,10 +362,14 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
   int* wlo = (int*)(tab + kTabLds);                // [P]
 
   const int tid = threadIdx.x;
+  // item = (pulse group fastest, then column chunk, then row tile)
   const int item = xcd_item(blockIdx.x, n_items);
   const int pstart = d.pulse0 + (item % n_pgroups) * P;
   const int pend = d.pulse0 + d.npulses;
-  const int y0 = (item / n_pgroups) * R;
+  const int rest = item / n_pgroups;
+  const int x_lo = (rest % n_xchunks) * CX;
+  const int x_hi = min(d.nx, x_lo + CX);
+  const int y0 = (rest / n_xchunks) * R;
   const int rows = min(R, d.ny - y0);
   const int nch = d.n_ch;
 
@@ -369,7 +386,8 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
     const double a0 = d.y_row[y0] - az_now, a1 = d.y_row[y0 + rows - 1] - az_now;
     double mmin = fmin((a0 / 2) * (a0 / d.sr0), (a1 / 2) * (a1 / d.sr0));
     if (a0 <= 0. && a1 >= 0.) mmin = 0.;
-    const double lo = (d.sr_col_min + mmin - d.resid_max - d.min_sr) / d.binsize;
+    const double sr_lo = n_xchunks > 1 ? d.sr_col[x_lo] : d.sr_col_min;
+    const double lo = (sr_lo + mmin - d.resid_max - d.min_sr) / d.binsize;
     wlo[tid] = max((int)floor(lo) - 5, 0);
   }
   __syncthreads();
@@ -440,15 +458,16 @@ __global__ __launch_bounds__(kThreads) void echo_accumulate_kernel(osar_echo_des
     // are in flight while column x is processed
     FacetIn<NPOL, SPEC> cur, nxt;
     FacetIdx ia = {0, 0}, ib = {0, 0};
-    load_index(d, (size_t)yrow, bragg_ok, ia);
-    load_facet<NPOL, SPEC, XTI>(d, 0, (size_t)yrow, snap_off + yrow, plane, rk, p, rslot,
+    const size_t idx_lo = (size_t)x_lo * d.ny + yrow;
+    load_index(d, idx_lo, bragg_ok, ia);
+    load_facet<NPOL, SPEC, XTI>(d, x_lo, idx_lo, snap_off + idx_lo, plane, rk, p, rslot,
                                 bragg_ok, ia, cur);
-    if (d.nx > 1) load_index(d, (size_t)d.ny + yrow, bragg_ok, ia);
-    for (int x = 0; x < d.nx; ++x) {
+    if (x_lo + 1 < x_hi) load_index(d, idx_lo + d.ny, bragg_ok, ia);
+    for (int x = x_lo; x < x_hi; ++x) {
       const size_t idx = (size_t)x * d.ny + yrow;
-      if (x + 1 < d.nx) {
+      if (x + 1 < x_hi) {
         ib = ia;
-        if (x + 2 < d.nx) load_index(d, idx + 2 * (size_t)d.ny, bragg_ok, ia);
+        if (x + 2 < x_hi) load_index(d, idx + 2 * (size_t)d.ny, bragg_ok, ia);
         load_facet<NPOL, SPEC, XTI>(d, x + 1, idx + d.ny, snap_off + idx + d.ny, plane, rk, p,
                                     rslot, bragg_ok, ib, nxt);
       }
@@ -629,10 +648,13 @@ int launch_accumulate(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipS
     if (rc) return rc;
   }
   const int n_pgroups = (d->npulses + P - 1) / P;
-  const int64_t n_items = (int64_t)n_pgroups * ((d->ny + R - 1) / R);
+  const int CX = (d->cols_per_block > 0 && d->cols_per_block < d->nx) ? d->cols_per_block : d->nx;
+  const int n_xchunks = (d->nx + CX - 1) / CX;
+  const int64_t n_items = (int64_t)n_pgroups * n_xchunks * ((d->ny + R - 1) / R);
   OSAR_REQUIRE(n_items < INT_MAX, OSAR_EINVAL, "too many workgroups (%lld)", (long long)n_items);
   hipLaunchKernelGGL((echo_accumulate_kernel<P, NCH, NPOL, SPEC, XTI>), dim3((unsigned)n_items),
-                     dim3(kThreads), lds, s, *d, W, R, LS, n_pgroups, (int)n_items);
+                     dim3(kThreads), lds, s, *d, W, R, LS, n_pgroups, n_xchunks, CX,
+                     (int)n_items);
   OSAR_LAUNCHED("echo_accumulate_kernel");
   return OSAR_OK;
 }
@@ -738,8 +760,8 @@ extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void
   int P = d->pulses_per_block ? d->pulses_per_block : 8;
   const int R = d->rows_per_block ? d->rows_per_block : 64;
   int W = d->window_bins ? d->window_bins : 512;
-  OSAR_REQUIRE(R >= 1 && R <= 1024 && W >= 8 && W <= 65536, OSAR_EINVAL,
-               "bad launch shape R=%d W=%d", R, W);
+  OSAR_REQUIRE(R >= 1 && R <= 1024 && W >= 8 && W <= 65536 && d->cols_per_block >= 0,
+               OSAR_EINVAL, "bad launch shape R=%d W=%d cols=%d", R, W, d->cols_per_block);
   hipStream_t s = (hipStream_t)stream;
   OSAR_HIP(hipSetDevice(ctx->device));
   switch (P) {

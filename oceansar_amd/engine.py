@@ -85,16 +85,25 @@ class RunState:
 
 
 class LaunchShape:
-    """Echo-accumulator launch shape: pulses per workgroup, rows per workgroup, LDS window."""
+    """Echo-accumulator launch shape: pulses per workgroup, rows per workgroup, LDS
+    window, columns per workgroup (0 = all)."""
 
-    def __init__(self, pulses_per_block=0, rows_per_block=0, window_bins=0):
+    def __init__(self, pulses_per_block=0, rows_per_block=0, window_bins=0, cols_per_block=0):
         self.pulses_per_block = pulses_per_block
         self.rows_per_block = rows_per_block
         self.window_bins = window_bins
+        self.cols_per_block = cols_per_block
+
+    def as_list(self):
+        return [self.pulses_per_block, self.rows_per_block, self.window_bins,
+                self.cols_per_block]
 
     def __repr__(self):
-        return 'LaunchShape(P=%d, R=%d, W=%d)' % (self.pulses_per_block, self.rows_per_block,
-                                                 self.window_bins)
+        s = 'LaunchShape(P=%d, R=%d, W=%d' % (self.pulses_per_block, self.rows_per_block,
+                                            self.window_bins)
+        if self.cols_per_block:
+            s += ', C=%d' % self.cols_per_block
+        return s + ')'
 
 
 class DeviceScene:
@@ -140,6 +149,7 @@ class DeviceScene:
         n_snap = sc.az_steps // sc.K + 2
         self.snap_times = _t([sc.snapshot_time(m) for m in range(n_snap)], f64, dev)
         self.resid_max = self._resid_bound()
+        self.sr_monotonic = bool(np.all(np.diff(sc.sr) >= 0))
         self.shape = shape if shape is not None else self.default_shape(lds_budget)
 
     # ------------------------------------------------------------------
@@ -160,12 +170,21 @@ class DeviceScene:
         cur = float(np.hypot(*self.sc.surface.current)) * self.n_pulses * self.sc.t_step
         return 8. * rms + cur + 0.5
 
-    def window_bins(self, rows):
-        """LDS window covering a tile of `rows` rows at any pulse (see echo.hip)."""
+    def window_bins(self, rows, cols=0):
+        """LDS window covering a tile of `rows` rows (and `cols` columns, 0 = all)
+        at any pulse (see echo.hip)."""
         sc = self.sc
         az_max = max(abs(sc.az_now(0)), abs(sc.az_now(sc.az_steps))) + sc.surface.Ly
         mig_spread = rows * float(sc.surface.dy) * az_max / sc.sr0
-        span = (sc.sr.max() - sc.sr.min() + mig_spread + 2 * self.resid_max) / sc.binsize
+        if cols and cols < sc.Nx:
+            if not self.sr_monotonic:
+                raise ValueError('column chunks need a non-decreasing slant-range table')
+            lo = sc.sr[0:sc.Nx:cols]
+            hi = sc.sr[np.minimum(np.arange(0, sc.Nx, cols) + cols, sc.Nx) - 1]
+            sr_span = float(np.max(hi - lo))
+        else:
+            sr_span = float(sc.sr.max() - sc.sr.min())
+        span = (sr_span + mig_spread + 2 * self.resid_max) / sc.binsize
         W = int(math.ceil(span)) + 8 + 6
         return min(((W + 15) // 16) * 16, ((sc.rg_samp + 15) // 16) * 16)
 
@@ -177,8 +196,21 @@ class DeviceScene:
         return 8 * npol * nch_c * P * LS + 4 * 164 + 4 * P
 
     def default_shape(self, lds_budget=160 * 1024):
-        """Largest P (pulses per workgroup) whose LDS lines fit; R = rows one pass covers."""
+        """The plain kernels are built for two workgroups per CU (4 waves per SIMD):
+        the largest P in (8, 4) and the widest column chunk whose LDS lines fit
+        twice in a CU (P = 8 loads each facet once for 8 pulses); otherwise
+        (specular, XTI, or no fit) the largest P whose lines fit once."""
         sc = self.sc
+        if not (sc.spec_enable or np.any(sc.b_xti != 0)):
+            for P in (8, 4):
+                R = min(8 * (64 // P), sc.Ny)
+                for cols in (sc.Nx, sc.Nx // 2, sc.Nx // 4, sc.Nx // 8):
+                    if cols < 64 or (cols < sc.Nx and not self.sr_monotonic):
+                        break
+                    c = cols if cols < sc.Nx else 0
+                    W = self.window_bins(R, c)
+                    if self.lds_bytes(P, sc.num_ch, self.npol, W) <= lds_budget // 2:
+                        return LaunchShape(P, R, W, c)
         for P in (8, 4, 2):
             R = min(8 * (64 // P), sc.Ny)      # 8 waves x 64/P rows
             W = self.window_bins(R)
@@ -284,6 +316,7 @@ class DeviceScene:
         d.pulses_per_block = shape.pulses_per_block
         d.rows_per_block = shape.rows_per_block
         d.window_bins = shape.window_bins
+        d.cols_per_block = shape.cols_per_block
         d.t_step, d.t_span, d.v_ground = sc.t_step, sc.t_span, sc.v_ground
         d.sr0, d.gr0, d.k0 = sc.sr0, sc.gr0, sc.k0
         d.min_sr, d.binsize = sc.min_sr, sc.sr_res / sc.over_fs

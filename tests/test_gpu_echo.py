@@ -115,8 +115,8 @@ def test_accumulator_matches_oracle(need_gpu, cfg1_scene):
 
 
 def test_accumulator_launch_shapes_agree(need_gpu, cfg1_scene):
-    """Different (P, R, W) decompositions give the same lines (small windows force the
-    global-memory fallback path)."""
+    """Different (P, R, W, columns) decompositions give the same lines (small windows
+    force the global-memory fallback path)."""
     import torch
     from oceansar_amd.engine import DeviceScene, LaunchShape
     sc = cfg1_scene
@@ -130,8 +130,11 @@ def test_accumulator_launch_shapes_agree(need_gpu, cfg1_scene):
         pr = int(sc.rcs_refresh[r])
         ds.bragg(dev['Diffx'][pr // sc.K - m0], dev['Diffy'][pr // sc.K - m0], pr, amp[r - r0])
     outs = []
-    for shape in (ds.shape, LaunchShape(2, 7, 64), LaunchShape(4, 256, 32), LaunchShape(2, 16, 128),
-                  LaunchShape(8, 100, 96)):
+    # column chunks: the window follows the chunk's first column (sr_col is monotonic)
+    shapes = (ds.shape, LaunchShape(2, 7, 64), LaunchShape(4, 256, 32), LaunchShape(2, 16, 128),
+              LaunchShape(8, 100, 96), LaunchShape(4, 128, ds.window_bins(128, 64), 64),
+              LaunchShape(2, 32, ds.window_bins(32, 100), 100), LaunchShape(4, 64, 24, 50))
+    for shape in shapes:
         raw = ds.new_raw()
         ds.accumulate(dev['Dx'], dev['Dy'], dev['Dz'], m0, amp, r0, p0, n, raw, shape=shape)
         outs.append(raw[:, :, p0:p0 + n].cpu().numpy().astype(np.complex128))
