@@ -88,6 +88,13 @@ __device__ __forceinline__ double2 mult(int m, double kx, double ky, double ik) 
   }
 }
 
+// mA(k) + i mB(k): the multiplier of the packed spectrum of plan entry (ma, mb)
+__device__ __forceinline__ double2 packed_mult(int ma, int mb, double kx, double ky, double ik) {
+  const double2 a = ma == M_NONE ? make_double2(0., 0.) : mult(ma, kx, ky, ik);
+  const double2 b = mb == M_NONE ? make_double2(0., 0.) : mult(mb, kx, ky, ik);
+  return make_double2(a.x - b.y, a.y + b.x);
+}
+
 // H(m a)[k] = (m(k) a(k) + conj(m(-k) a(-k))) / 2
 __device__ __forceinline__ double2 herm(int m, double kx, double ky, double ik, double kxm,
                                         double kym, double ikm, double2 a, double2 am) {
@@ -347,14 +354,25 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
       if (i >= nx) break;
       const int im = (nx - i) & (nx - 1);
       const double kx = kx0[u], kxm = kx1[u];
-      // S at k = (j, i)
-      const double2 ha = herm(q.ma, kx, kyj, ik0[u], kxm, kyjm, ik1[u], A0[u], A1[u]);
-      const double2 hb = herm(q.mb, kx, kyj, ik0[u], kxm, kyjm, ik1[u], A0[u], A1[u]);
-      S[i] = make_double2(ha.x - hb.y, ha.y + hb.x);  // ha + i hb
-      // S at -k = (jm, im); for a self-paired row this rewrites the same value
-      const double2 hc = herm(q.ma, kxm, kyjm, ik1[u], kx, kyj, ik0[u], A1[u], A0[u]);
-      const double2 hd = herm(q.mb, kxm, kyjm, ik1[u], kx, kyj, ik0[u], A1[u], A0[u]);
-      S[(nrow - 1) * nx + im] = make_double2(hc.x - hd.y, hc.y + hd.x);
+      double2 sk, smk;   // S at k = (j, i) and at -k = (jm, im)
+      if (kxm == -kx && kyjm == -kyj) {
+        // Every multiplier has m(-k) = conj(m(k)) where -k is the true mirror,
+        // so H(m a) = m H(a): one H(a) serves every packed spectrum.
+        const double2 h = make_double2(0.5 * (A0[u].x + A1[u].x), 0.5 * (A0[u].y - A1[u].y));
+        sk = cmul2(packed_mult(q.ma, q.mb, kx, kyj, ik0[u]), h);
+        smk = cmul2(packed_mult(q.ma, q.mb, kxm, kyjm, ik1[u]), make_double2(h.x, -h.y));
+      } else {
+        // a Nyquist row or column is its own mirror: the general form
+        const double2 ha = herm(q.ma, kx, kyj, ik0[u], kxm, kyjm, ik1[u], A0[u], A1[u]);
+        const double2 hb = herm(q.mb, kx, kyj, ik0[u], kxm, kyjm, ik1[u], A0[u], A1[u]);
+        sk = make_double2(ha.x - hb.y, ha.y + hb.x);  // ha + i hb
+        const double2 hc = herm(q.ma, kxm, kyjm, ik1[u], kx, kyj, ik0[u], A1[u], A0[u]);
+        const double2 hd = herm(q.mb, kxm, kyjm, ik1[u], kx, kyj, ik0[u], A1[u], A0[u]);
+        smk = make_double2(hc.x - hd.y, hc.y + hd.x);
+      }
+      S[i] = sk;
+      // for a self-paired row this rewrites the value the mirror's thread writes
+      S[(nrow - 1) * nx + im] = smk;
     }
     __syncthreads();
     lds_ifft<kRowThreads, 2 * kMaxN / kRowThreads, 3, false>(S, nx, a.logx, nrow, 0, nx, 1, tw);
