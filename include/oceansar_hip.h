@@ -83,7 +83,7 @@ typedef struct osar_echo_desc {
   int32_t n_rcs_slots;      /* slots in amp */
   int32_t flags;            /* OSAR_ECHO_* */
   /* launch shape (0 = library default) */
-  int32_t pulses_per_block; /* 1, 2, 4 or 8 */
+  int32_t pulses_per_block; /* 2, 4, 8 or 16 */
   int32_t rows_per_block;
   int32_t window_bins;      /* LDS range-line window per (pol, ch, pulse) */
   int32_t pol;              /* 0 = vv, 1 = hh, 2 = DP (line 0 hh, line 1 vv) */

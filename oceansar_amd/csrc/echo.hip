@@ -768,9 +768,10 @@ extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void
     case 2: return launch_ch<2>(ctx, d, W, R, s);
     case 4: return launch_ch<4>(ctx, d, W, R, s);
     case 8: return launch_ch<8>(ctx, d, W, R, s);
+    case 16: return launch_ch<16>(ctx, d, W, R, s);
     default: break;
   }
-  osar_set_error("pulses_per_block must be 2, 4 or 8 (got %d)", P);
+  osar_set_error("pulses_per_block must be 2, 4, 8 or 16 (got %d)", P);
   return OSAR_EINVAL;
 }
 

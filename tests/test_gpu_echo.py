@@ -133,7 +133,8 @@ def test_accumulator_launch_shapes_agree(need_gpu, cfg1_scene):
     # column chunks: the window follows the chunk's first column (sr_col is monotonic)
     shapes = (ds.shape, LaunchShape(2, 7, 64), LaunchShape(4, 256, 32), LaunchShape(2, 16, 128),
               LaunchShape(8, 100, 96), LaunchShape(4, 128, ds.window_bins(128, 64), 64),
-              LaunchShape(2, 32, ds.window_bins(32, 100), 100), LaunchShape(4, 64, 24, 50))
+              LaunchShape(2, 32, ds.window_bins(32, 100), 100), LaunchShape(4, 64, 24, 50),
+              LaunchShape(16, 32, ds.window_bins(32, 64), 64), LaunchShape(16, 40, 40, 0))
     for shape in shapes:
         raw = ds.new_raw()
         ds.accumulate(dev['Dx'], dev['Dy'], dev['Dz'], m0, amp, r0, p0, n, raw, shape=shape)
