@@ -185,6 +185,7 @@ class BatchRunner:
     def _rewind(self, st):
         st.p = st.p0
         st.held = -1
+        st.snap_held = -1
 
     def _zero(self, ds, st):
         ds.zero_c32(st.raw)

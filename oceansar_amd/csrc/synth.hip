@@ -66,6 +66,35 @@ struct SynthArgs {
   QPlan q[kMaxQ];
 };
 
+// Plan entry / output pointer at a wave-uniform runtime index. A switch keeps
+// the by-value kernel argument in scalar registers (indexing its arrays with a
+// runtime value would copy the whole argument to scratch memory).
+__device__ __forceinline__ QPlan plan_at(const SynthArgs& a, int i) {
+  switch (i) {
+    case 0: return a.q[0];
+    case 1: return a.q[1];
+    case 2: return a.q[2];
+    case 3: return a.q[3];
+    case 4: return a.q[4];
+    case 5: return a.q[5];
+    case 6: return a.q[6];
+    default: return a.q[7];
+  }
+}
+
+__device__ __forceinline__ float* out_at(const SynthArgs& a, int i) {
+  switch (i) {
+    case 0: return a.out[0];
+    case 1: return a.out[1];
+    case 2: return a.out[2];
+    case 3: return a.out[3];
+    case 4: return a.out[4];
+    case 5: return a.out[5];
+    case 6: return a.out[6];
+    default: return a.out[7];
+  }
+}
+
 __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -319,10 +348,12 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
   double2 A0[kRowPairs], A1[kRowPairs];   // a(k) at (j, i) and a(-k) at (jm, -i)
   double ik0[kRowPairs], ik1[kRowPairs];  // 1/|k| (0 off the good-k set)
   double kx0[kRowPairs], kx1[kRowPairs];  // kx at columns i and -i
+  // Every element of the register arrays is assigned on every path (columns
+  // beyond nx are clamped and their results never stored), so the arrays stay
+  // in registers.
 #pragma unroll
   for (int u = 0; u < kRowPairs; ++u) {
-    const int i = tid + u * kRowThreads;
-    if (i >= nx) break;
+    const int i = min(tid + u * kRowThreads, nx - 1);
     const int im = (nx - i) & (nx - 1);
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
@@ -347,11 +378,11 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
     }
   }
   for (int qi = 0; qi < a.nq; ++qi) {
-    const QPlan q = a.q[qi];
+    const QPlan q = plan_at(a, qi);
 #pragma unroll
     for (int u = 0; u < kRowPairs; ++u) {
-      const int i = tid + u * kRowThreads;
-      if (i >= nx) break;
+      const bool in = tid + u * kRowThreads < nx;
+      const int i = min(tid + u * kRowThreads, nx - 1);
       const int im = (nx - i) & (nx - 1);
       const double kx = kx0[u], kxm = kx1[u];
       double2 sk, smk;   // S at k = (j, i) and at -k = (jm, im)
@@ -370,9 +401,11 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
         const double2 hd = herm(q.mb, kxm, kyjm, ik1[u], kx, kyj, ik0[u], A1[u], A0[u]);
         smk = make_double2(hc.x - hd.y, hc.y + hd.x);
       }
-      S[i] = sk;
-      // for a self-paired row this rewrites the value the mirror's thread writes
-      S[(nrow - 1) * nx + im] = smk;
+      if (in) {
+        S[i] = sk;
+        // for a self-paired row this rewrites the value the mirror's thread writes
+        S[(nrow - 1) * nx + im] = smk;
+      }
     }
     __syncthreads();
     lds_ifft<kRowThreads, 2 * kMaxN / kRowThreads, 3, false>(S, nx, a.logx, nrow, 0, nx, 1, tw);
@@ -387,6 +420,9 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
   }
 }
 
+// Pass 2: one workgroup per (packed spectrum, block of cb columns, snapshot):
+// the block is one contiguous run of the workspace; FFT along y in LDS, then
+// scaled by 1/(Nx Ny) and written as float32 fields, x-major (field[x][y]).
 // Pass 2: one workgroup per (packed spectrum, block of cb columns, snapshot):
 // the block is one contiguous run of the workspace; FFT along y in LDS, then
 // scaled by 1/(Nx Ny) and written as float32 fields, x-major (field[x][y]).
@@ -406,30 +442,32 @@ __global__ __launch_bounds__(kColThreads) void synth_cols_kernel(SynthArgs a) {
   for (int e = tid; e < ny * cb; e += kColThreads) C[e] = w[e];
   __syncthreads();
   lds_ifft<kColThreads, kColElems / kColThreads, 4, true>(C, ny, a.logy, cb, a.lcb, 1, cb, tw);
-  const QPlan q = a.q[qi];
+  const QPlan q = plan_at(a, qi);
   const double scale = 1.0 / ((double)nx * (double)ny);
   const double t = a.times[ti];
   const size_t plane = (size_t)nx * ny;
+  float* const outa = q.outa >= 0 ? out_at(a, q.outa) : nullptr;
+  float* const outb = q.outb >= 0 ? out_at(a, q.outb) : nullptr;
   for (int e = tid; e < ny * cb; e += kColThreads) {
     const int c = e >> a.logy, y = e & (ny - 1);  // consecutive threads walk y: contiguous stores
     const double2 z = C[y * cb + c];
     const double re = z.x * scale, im = z.y * scale;
     const size_t o = (size_t)ti * plane + (size_t)(x0 + c) * ny + y;
     if (q.choppy) {
-      a.out[q.outa][o] = (float)(re / (1. + im));
+      outa[o] = (float)(re / (1. + im));
       continue;
     }
     if (q.outa >= 0) {
       double v = re;
       if (q.outa == 0) v = v + a.cur_x * t;
       if (q.outa == 1) v = v + a.cur_y * t;
-      a.out[q.outa][o] = (float)v;
+      outa[o] = (float)v;
     }
     if (q.outb >= 0) {
       double v = im;
       if (q.outb == 0) v = v + a.cur_x * t;
       if (q.outb == 1) v = v + a.cur_y * t;
-      a.out[q.outb][o] = (float)v;
+      outb[o] = (float)v;
     }
   }
 }

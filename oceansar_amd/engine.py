@@ -399,6 +399,8 @@ class DeviceScene:
             st.amp = torch.empty((st.C + 2, self.npol, 2, nx, ny), dtype=torch.float32,
                                  device=self.dev)
         st.held = -1          # refresh index whose amplitudes sit in amp[0]
+        st.snap_held = -1     # snapshot index held in buf slot st.snap_slot
+        st.snap_slot = 0
         st.spec_energy = None
         if sc.spec_enable:
             st.spec_energy = torch.zeros(self.n_pulses, dtype=torch.float64, device=self.dev)
@@ -420,7 +422,15 @@ class DeviceScene:
         last = q1 - 1
         m_last = last // K + (1 if last % K else 0)
         ev = self._events(timer)
-        self.synth(self.snap_times[m0:m_last + 1], buf, st.names)
+        if st.snap_held == m0 and m_last > m0:
+            # the previous chunk's last snapshot is this chunk's first: keep it
+            for n in st.names:
+                buf[n][0].copy_(buf[n][st.snap_slot])
+            self.synth(self.snap_times[m0 + 1:m_last + 1], {n: buf[n][1:] for n in st.names},
+                       st.names)
+        else:
+            self.synth(self.snap_times[m0:m_last + 1], buf, st.names)
+        st.snap_held, st.snap_slot = m_last, m_last - m0
         self._mark(timer, 'synth', ev)
         r0 = r1 = 0
         if sc.bragg_enable:
