@@ -187,6 +187,10 @@ def main():
     ap.add_argument('--chunk', type=int, default=32, help='snapshot intervals per launch group')
     ap.add_argument('--cpu-sample', type=int, default=96)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--overlap', action='store_true',
+                    help='two streams: synthesis/Bragg of chunk c+1 beside the accumulation '
+                         'of chunk c (measured no faster on the large workload: every stage '
+                         'already fills the GPU)')
     ap.add_argument('--form', default='plain', choices=('plain', 'batch', 'serve'),
                     help='plain: one scene (the headline line); batch: 64 mixed scenes, '
                          'plain vs concurrent vs one graph launch; serve: Poisson job '
@@ -218,9 +222,10 @@ def main():
     ds = DeviceScene(sc)
     W = sc.contributions
     raw = ds.new_raw()
+    overlap = args.overlap
     for _ in range(args.warmup):
         ds.zero_c32(raw)
-        ds.run(raw=raw, chunk_snaps=args.chunk)
+        ds.run(raw=raw, chunk_snaps=args.chunk, overlap=overlap)
     torch.cuda.synchronize()
 
     stage_events = []
@@ -232,7 +237,7 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ds.zero_c32(raw)
-        ds.run(raw=raw, chunk_snaps=args.chunk, timer=timer)
+        ds.run(raw=raw, chunk_snaps=args.chunk, timer=timer, overlap=overlap)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
 

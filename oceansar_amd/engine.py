@@ -364,21 +364,24 @@ class DeviceScene:
                                   times, out, fields)
 
     # ------------------------------------------------------------------
-    def run(self, p0=0, p1=None, chunk_snaps=32, raw=None, shape=None, timer=None):
+    def run(self, p0=0, p1=None, chunk_snaps=32, raw=None, shape=None, timer=None,
+            overlap=False):
         """GPU raw cube for pulses [p0, p1): returns (raw device tensor, NRCS_avg numpy).
 
-        Snapshots are synthesised in chunks of `chunk_snaps` intervals (the
-        boundary snapshot of a chunk is recomputed by the next one), Bragg
+        Snapshots are synthesised in chunks of `chunk_snaps` intervals, Bragg
         amplitudes of every refresh of the chunk in one launch, then one
-        accumulator launch covers the chunk. `timer`, if given, is called as
-        timer(stage, start_event, end_event) for every launch group.
+        accumulator launch covers the chunk. With `overlap`, synthesis and
+        Bragg of chunk c + 1 run on one stream while the accumulation of chunk
+        c runs on another (double-buffered snapshots and amplitudes). `timer`,
+        if given, is called as timer(stage, start_event, end_event) for every
+        launch group.
         """
-        st = self.start(p0, p1, chunk_snaps, raw, shape)
+        st = self.start(p0, p1, chunk_snaps, raw, shape, overlap=overlap)
         while self.step(st, timer):
             pass
         return self.finish(st)
 
-    def start(self, p0=0, p1=None, chunk_snaps=32, raw=None, shape=None):
+    def start(self, p0=0, p1=None, chunk_snaps=32, raw=None, shape=None, overlap=False):
         """Allocate the buffers of a run over pulses [p0, p1) (no launches)."""
         import torch
         sc = self.sc
@@ -390,30 +393,62 @@ class DeviceScene:
         st.raw = self.new_raw() if raw is None else raw
         st.names = self.fields_needed()
         nx, ny = sc.Nx, sc.Ny
-        st.buf = {n: torch.empty((st.C + 1, nx, ny), dtype=torch.float32, device=self.dev)
-                  for n in st.names}
+        nbuf = 2 if overlap else 1
+        st.bufs = [{n: torch.empty((st.C + 1, nx, ny), dtype=torch.float32, device=self.dev)
+                    for n in st.names} for _ in range(nbuf)]
         nref = len(sc.rcs_refresh) if sc.bragg_enable else 0
         st.rcs_sum = torch.zeros(max(nref, 1), self.npol, dtype=torch.float64, device=self.dev)
-        st.amp = None
+        st.amps = [None] * nbuf
         if sc.bragg_enable:
-            st.amp = torch.empty((st.C + 2, self.npol, 2, nx, ny), dtype=torch.float32,
-                                 device=self.dev)
-        st.held = -1          # refresh index whose amplitudes sit in amp[0]
-        st.snap_held = -1     # snapshot index held in buf slot st.snap_slot
+            st.amps = [torch.empty((st.C + 2, self.npol, 2, nx, ny), dtype=torch.float32,
+                                   device=self.dev) for _ in range(nbuf)]
+        st.k = 0                # buffer set of the next chunk
+        st.held = -1            # refresh index whose amplitudes sit in slot held_slot
+        st.held_slot = 0
+        st.snap_held = -1       # snapshot index held in slot snap_slot
         st.snap_slot = 0
         st.spec_energy = None
         if sc.spec_enable:
             st.spec_energy = torch.zeros(self.n_pulses, dtype=torch.float64, device=self.dev)
+        st.streams = None
+        if overlap:
+            cur = torch.cuda.current_stream(self.dev)
+            st.streams = (torch.cuda.Stream(device=self.dev), torch.cuda.Stream(device=self.dev))
+            for sx in st.streams:
+                sx.wait_stream(cur)
+            st.echo_done = [None] * nbuf     # last accumulation that read buffer set k
         return st
 
     def step(self, st, timer=None):
-        """Enqueue the next chunk of `st` on the current stream; False when done."""
+        """Enqueue the next chunk of `st` (on the current stream, or on the run's
+        two streams with overlap); False when done."""
         if st.p >= st.p1:
             return False
         import torch
+        if st.streams is None:
+            self._step(st, timer, None, None)
+            return True
+        sa, sb = st.streams
+        k = st.k
+        with torch.cuda.stream(sa):
+            if st.echo_done[k] is not None:
+                sa.wait_event(st.echo_done[k])       # the accumulation two chunks back
+            ready = self._step(st, timer, sa, sb)
+        with torch.cuda.stream(sb):
+            done = self._step_echo(st, timer, ready)
+        st.echo_done[k] = done
+        return True
+
+    def _step(self, st, timer, sa, sb):
+        """Synthesis and Bragg of the next chunk into buffer set st.k (plus the
+        accumulation when not overlapped). Returns the event the accumulation
+        waits for (overlap) or None."""
+        import torch
         sc = self.sc
         K = sc.K
-        p, p1, C, buf = st.p, st.p1, st.C, st.buf
+        p, p1, C = st.p, st.p1, st.C
+        prev = st.k if len(st.bufs) == 1 else 1 - st.k
+        buf, amp = st.bufs[st.k], st.amps[st.k]
         m0 = p // K
         m1 = min(m0 + C, (p1 - 1) // K + 1)       # last snapshot needed
         q1 = min(m1 * K, p1) if m1 > m0 else p1
@@ -424,8 +459,9 @@ class DeviceScene:
         ev = self._events(timer)
         if st.snap_held == m0 and m_last > m0:
             # the previous chunk's last snapshot is this chunk's first: keep it
+            src = st.bufs[prev]
             for n in st.names:
-                buf[n][0].copy_(buf[n][st.snap_slot])
+                buf[n][0].copy_(src[n][st.snap_slot])
             self.synth(self.snap_times[m0 + 1:m_last + 1], {n: buf[n][1:] for n in st.names},
                        st.names)
         else:
@@ -437,35 +473,63 @@ class DeviceScene:
             r0 = int(sc.rcs_slot[p])
             r1 = int(sc.rcs_slot[last])
             ev = self._events(timer)
-            r_first = r0 + 1 if r0 == st.held else r0
+            r_first = r0
+            if r0 == st.held:
+                # the last refresh of the previous chunk is still in force
+                if st.held_slot != 0 or prev != st.k:
+                    amp[0].copy_(st.amps[prev][st.held_slot])
+                r_first = r0 + 1
             if r_first == r0 and int(sc.rcs_refresh[r0]) // K < m0:
                 # refresh pulse before this chunk: synthesise its slopes alone
                 pr = int(sc.rcs_refresh[r0])
                 tmp = {n: torch.empty((1, sc.Nx, sc.Ny), dtype=torch.float32, device=self.dev)
                        for n in ('Diffx', 'Diffy')}
                 self.synth(self.snap_times[pr // K:pr // K + 1], tmp, ['Diffx', 'Diffy'])
-                self.bragg(tmp['Diffx'][0], tmp['Diffy'][0], pr, st.amp[0], st.rcs_sum[r0])
+                self.bragg(tmp['Diffx'][0], tmp['Diffy'][0], pr, amp[0], st.rcs_sum[r0])
                 r_first += 1
             if r_first <= r1:
                 self.bragg_refreshes(r_first, r1, buf['Diffx'], buf['Diffy'], m0,
-                                     st.amp[r_first - r0:], st.rcs_sum)
+                                     amp[r_first - r0:], st.rcs_sum)
             self._mark(timer, 'bragg', ev)
+            st.held, st.held_slot = r1, r1 - r0
+        st.chunk = (p, q1, m0, r0)
+        st.p = q1
+        if sa is None:
+            self._step_echo(st, timer, None)
+            return None
+        ready = torch.cuda.Event()
+        ready.record(sa)
+        return ready
+
+    def _step_echo(self, st, timer, ready):
+        """The accumulation of the chunk prepared by _step (buffer set st.k)."""
+        import torch
+        sc = self.sc
+        p, q1, m0, r0 = st.chunk
+        buf, amp = st.bufs[st.k], st.amps[st.k]
+        if ready is not None:
+            torch.cuda.current_stream(self.dev).wait_event(ready)
         ev = self._events(timer)
-        self.accumulate(buf['Dx'], buf['Dy'], buf['Dz'], m0, st.amp, r0, p, q1 - p, st.raw,
+        self.accumulate(buf['Dx'], buf['Dy'], buf['Dz'], m0, amp, r0, p, q1 - p, st.raw,
                         st.shape, spec=buf if sc.spec_enable else None,
                         spec_energy=st.spec_energy)
         self._mark(timer, 'echo', ev)
-        if sc.bragg_enable:
-            # the last refresh in force stays in amp[0] for the next chunk
-            if r1 > r0:
-                st.amp[0].copy_(st.amp[r1 - r0])
-            st.held = r1
-        st.p = q1
-        return True
+        done = None
+        if ready is not None:
+            done = torch.cuda.Event()
+            done.record(torch.cuda.current_stream(self.dev))
+        if len(st.bufs) > 1:
+            st.k = 1 - st.k
+        return done
 
     def finish(self, st):
         """Wait for the run and return (raw device tensor, NRCS_avg numpy [pol][pulse])."""
+        import torch
         sc = self.sc
+        if st.streams is not None:
+            cur = torch.cuda.current_stream(self.dev)
+            for sx in st.streams:
+                cur.wait_stream(sx)
         p0, p1 = st.p0, st.p1
         nrcs = np.zeros((self.npol, self.n_pulses))
         if sc.bragg_enable:
