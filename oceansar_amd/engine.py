@@ -132,8 +132,12 @@ class DeviceScene:
             self.scat_m = _t(self._extend(sc.scat_m), c64, dev)
             self.scat_p = _t(self._extend(sc.scat_p), c64, dev)
             # x-major [nx][ny], like every per-facet device array
-            self.ind0_m = _t(np.mod(sc.scat_m.ind0, sc.scat_m.N).T, i32, dev)
-            self.ind0_p = _t(np.mod(sc.scat_p.ind0, sc.scat_p.N).T, i32, dev)
+            # uploaded row-major, transposed on the device (a host transpose of a
+            # 2048^2 table costs ~0.1 s)
+            self.ind0_m = _t(np.mod(sc.scat_m.ind0, sc.scat_m.N).astype(np.int32), i32,
+                             dev).t().contiguous()
+            self.ind0_p = _t(np.mod(sc.scat_p.ind0, sc.scat_p.N).astype(np.int32), i32,
+                             dev).t().contiguous()
             self.rcs_slot = _t(sc.rcs_slot, i32, dev)
             if np.any(sc.rcs_refresh % sc.K):
                 raise ValueError('Bragg refresh pulses must be snapshot pulses (multiples of K)')
@@ -164,9 +168,11 @@ class DeviceScene:
 
         Only sizes the LDS window; facets beyond it take the global-memory path.
         """
-        c = self.sc.surface.wave_coefs.astype(np.complex128)
+        import torch
         n2 = float(self.sc.Nx * self.sc.Ny)
-        rms = math.sqrt(float(np.sum(np.abs(c) ** 2)) / 2.) / n2
+        c = self.coefs        # device complex64 coefficients
+        power = float(torch.sum(c.real.double() ** 2 + c.imag.double() ** 2).item())
+        rms = math.sqrt(power / 2.) / n2
         cur = float(np.hypot(*self.sc.surface.current)) * self.n_pulses * self.sc.t_step
         return 8. * rms + cur + 0.5
 
