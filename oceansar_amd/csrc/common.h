@@ -80,6 +80,17 @@ __device__ __forceinline__ double div_by(double a, double b, double inv) {
   return fma(r, inv, q0);
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear block index,
+// and each XCD has its own L2. xcd_item maps block h of G to a work item so
+// that XCD x runs the contiguous item range [x G / 8, (x + 1) G / 8) in order:
+// neighbouring items run at the same time behind the same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_item(int h, int G) {
+  const int q = G / kXcds, r = G % kXcds;
+  const int x = h % kXcds, s = h / kXcds;
+  return x * q + min(x, r) + s;
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }

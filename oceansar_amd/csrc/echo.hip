@@ -336,16 +336,9 @@ __device__ __forceinline__ void flush_slot(const Lines& L, const float2 (&s)[NPO
 
 // XTI: the cross-track phase varies per facet, so channels cannot share a
 // window; this template variant deposits every tap of every channel.
-// Workgroups are dispatched to the 8 XCDs round-robin, and each XCD has its
-// own L2. Work item w = (pulse group fastest, then row tile) is remapped so
-// that each XCD runs a contiguous range of items: the pulse groups of one
-// snapshot interval, which read the same surface rows, then share an L2.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_item(int h, int G) {
-  const int q = G / kXcds, r = G % kXcds;
-  const int j = h % kXcds, s = h / kXcds;
-  return j * q + min(j, r) + s;
-}
+// Work item w = (pulse group fastest, then row tile) is remapped with
+// xcd_item so that each XCD runs a contiguous range of items: the pulse groups
+// of one snapshot interval, which read the same surface rows, then share an L2.
 
 template <int P, int NCH, int NPOL, bool SPEC, bool XTI>
 // Two workgroups per CU (4 waves per SIMD, at most 128 VGPRs) for the plain
@@ -363,7 +356,7 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
 
   const int tid = threadIdx.x;
   // item = (pulse group fastest, then column chunk, then row tile)
-  const int item = xcd_item(blockIdx.x, n_items);
+  const int item = osar::xcd_item(blockIdx.x, n_items);
   const int pstart = d.pulse0 + (item % n_pgroups) * P;
   const int pend = d.pulse0 + d.npulses;
   const int rest = item / n_pgroups;

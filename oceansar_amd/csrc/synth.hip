@@ -12,14 +12,17 @@
 // ifft2(S) = A + iB. A choppy slope uses the pair (numerator, denominator) and
 // divides in the epilogue.
 //
-// Pass 1 (rows): one workgroup per row pair (j, -j): phased coefficients of
-// both rows in registers (each thread owns columns and their mirrors), packed
-// spectra built into LDS, radix-4/2 Stockham FFT along x in LDS, rows written
-// to an FP64 workspace in column-blocked layout.
-// Pass 2 (columns): one workgroup per (packed spectrum, block of CB columns):
-// the block is read as one contiguous run, FFT along y in LDS, scaled by
-// 1/(Nx Ny) and written as float32 fields in x-major layout (field[x][y]) so
-// that every store is a contiguous run along y.
+// Pass 1 (rows): one workgroup per row pair (j, -j): H(a) of both rows in
+// registers (each thread owns columns and their mirrors), packed spectra built
+// into LDS, radix-8/4 Stockham FFT along x in LDS, rows written to an FP64
+// workspace W[q][x / cbw][y][x % cbw] (cbw = 8: each row writes full 128-byte
+// lines; neighbouring rows run on one XCD).
+// Pass 2 (columns): one workgroup per (block of cb = 2 columns, packed
+// spectrum): the block is read from its workspace runs (the blocks of one run
+// on one XCD), FFT along y in LDS, scaled by 1/(Nx Ny) and written as float32
+// fields in x-major layout (field[x][y]) so that every store is a contiguous
+// run along y. Both passes fit two workgroups per CU (LDS and registers), so
+// one workgroup's HBM traffic overlaps the other's FFT.
 #include <math.h>
 
 #include <vector>
@@ -50,10 +53,16 @@ struct QPlan {
   int ma, mb;      // multipliers of the real and imaginary component
   int outa, outb;  // output field index (0..7) or -1
   int choppy;      // 1: single output = Re / (1 + Im) into outa
+  int ca, cb;      // the multipliers as monomial codes (mono_code)
 };
 
+// Every multiplier is a monomial s i^d kx^px ky^py |k|^-pk. Code bits: px (0-1),
+// py (2-3), pk (4), d (5), s = -1 (6), no multiplier (7).
+constexpr int kMonoImag = 32, kMonoNeg = 64, kMonoNone = 128;
+
 struct SynthArgs {
-  int nx, ny, nq, logx, logy, cb, lcb;
+  int nx, ny, nq, logx, logy, cb, lcb, n_times;
+  int cbw, lcbw;   // workspace block width (columns per contiguous run of a row) and log2
   double depth, cur_x, cur_y, thr;
   const float* kx;
   const float* ky;
@@ -99,37 +108,36 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Spectral multiplier m(k) for wave vector (kx, ky) with |k| and 1/|k| given
-// (1/|k| = 0 and k/|k| = 0 off the good-k set, as ocean.py:141-148 does).
-__device__ __forceinline__ double2 mult(int m, double kx, double ky, double ik) {
-  switch (m) {
-    case M_ONE: return make_double2(1., 0.);
-    case M_IKXN: return make_double2(0., kx * ik);
-    case M_IKYN: return make_double2(0., ky * ik);
-    case M_IKX: return make_double2(0., kx);
-    case M_IKY: return make_double2(0., ky);
-    case M_NKX2: return make_double2(-(kx * kx), 0.);
-    case M_NKY2: return make_double2(-(ky * ky), 0.);
-    case M_NKXKY: return make_double2(-kx * ky, 0.);
-    case M_NKX2KI: return make_double2(-(kx * kx) * ik, 0.);
-    case M_NKY2KI: return make_double2(-(ky * ky) * ik, 0.);
-    default: return make_double2(0., 0.);
-  }
+// Spectral multiplier m(k) = s i^d kx^px ky^py |k|^-pk for wave vector
+// (kx, ky), with 1/|k| = 0 off the good-k set (so k/|k| = 0 there, as
+// ocean.py:141-148 does): the row-uniform factor s ky^py (0 for no
+// multiplier) times the per-column factor kx^px |k|^-pk; a sign or a unit
+// factor is exact, so m(-k) is bitwise conj(m(k)) on a mirror-symmetric grid.
+__device__ __forceinline__ double mono_y(int code, double ky) {
+  if (code & kMonoNone) return 0.;
+  const int py = (code >> 2) & 3;
+  double y = py >= 1 ? ky : 1.;
+  if (py == 2) y = y * ky;
+  return (code & kMonoNeg) ? -y : y;
 }
 
-// mA(k) + i mB(k): the multiplier of the packed spectrum of plan entry (ma, mb)
-__device__ __forceinline__ double2 packed_mult(int ma, int mb, double kx, double ky, double ik) {
-  const double2 a = ma == M_NONE ? make_double2(0., 0.) : mult(ma, kx, ky, ik);
-  const double2 b = mb == M_NONE ? make_double2(0., 0.) : mult(mb, kx, ky, ik);
-  return make_double2(a.x - b.y, a.y + b.x);
+__device__ __forceinline__ double mono_x(int code, double kx, double ik) {
+  const int px = code & 3;
+  double f = px >= 1 ? kx : 1.;
+  if (px == 2) f = f * kx;
+  if (code & 16) f = f * ik;
+  return f;
 }
 
-// H(m a)[k] = (m(k) a(k) + conj(m(-k) a(-k))) / 2
-__device__ __forceinline__ double2 herm(int m, double kx, double ky, double ik, double kxm,
-                                        double kym, double ikm, double2 a, double2 am) {
-  if (m == M_NONE) return make_double2(0., 0.);
-  const double2 g = cmul2(mult(m, kx, ky, ik), a);
-  const double2 gm = cmul2(mult(m, kxm, kym, ikm), am);
+// the multiplier value: v real, or i v when d
+__device__ __forceinline__ double2 mono_c(bool d, double v) {
+  return d ? make_double2(0., v) : make_double2(v, 0.);
+}
+
+// H(m a)[k] = (m(k) a(k) + conj(m(-k) a(-k))) / 2 with m(k) = mk, m(-k) = mm
+__device__ __forceinline__ double2 herm(double2 mk, double2 a, double2 mm, double2 am) {
+  const double2 g = cmul2(mk, a);
+  const double2 gm = cmul2(mm, am);
   return make_double2(0.5 * (g.x + gm.x), 0.5 * (g.y - gm.y));
 }
 
@@ -259,17 +267,15 @@ __device__ __forceinline__ void stockham_pass(double2* buf, int logN, int Ns, in
       const int j = SEQ_FAST ? (id >> lseq) : (id & (nb - 1));
       const int jm = j & (Ns - 1);
       const int step = jm << (logN - logNs - LR);   // jm * N / (Ns * R)
-      // w^(r step) as a balanced product tree of w^step (depth <= 2 log2 R)
-      double2 wr[R];
-      wr[0] = make_double2(1., 0.);
-      if (R > 1) wr[1] = twiddle(tw, step);
+      // w^(r step) by repeated multiplication with w^step (R - 2 products; the
+      // rounding error grows by about one ulp per product, far below float32)
+      const double2 w1 = twiddle(tw, step);
+      double2 wr = w1;
 #pragma unroll
-      for (int r = 2; r < R; ++r) {
-        const int hi = (r & (r - 1)) == 0 ? r / 2 : (1 << (31 - __builtin_clz(r)));
-        wr[r] = cmul2(wr[hi], wr[r - hi]);
+      for (int r = 1; r < R; ++r) {
+        v[b][r] = cmul2(v[b][r], wr);
+        if (r + 1 < R) wr = cmul2(wr, w1);
       }
-#pragma unroll
-      for (int r = 1; r < R; ++r) v[b][r] = cmul2(v[b][r], wr[r]);
       dft<R>(v[b]);
       const int o = (j - jm) * R + jm;
 #pragma unroll
@@ -321,33 +327,79 @@ __device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int lseq, int 
 }
 
 constexpr int kRowThreads = 512;
+constexpr int kRowWaves = 4;                     // waves per SIMD: two row workgroups per CU
 constexpr int kColThreads = 512;
-constexpr int kColElems = 8192;                  // cb * ny: elements of one column block
+constexpr int kColWaves = 4;                     // two column workgroups per CU
+constexpr int kColElems = 4096;                  // cb * ny: elements of one column block
+constexpr int kColRadixLog = 3;                  // radix-8 passes (one butterfly per thread)
+constexpr int kWsBlock = 8;                      // workspace run: 8 columns = 128 bytes
 constexpr int kRowPairs = kMaxN / kRowThreads;   // column pairs a row thread owns
+
+// a(k) exp(-i omega(|k|) t) for the complex64 coefficient c and wave number k
+__device__ __forceinline__ double2 phased(osar_c32 c, double k, double t, double depth) {
+  const double om = depth > 0. ? sqrt(kG * k * tanh(k * depth)) : sqrt(kG * k);
+  double sn, cs;
+  sincos(-(om * t), &sn, &cs);
+  return cmul2(make_double2((double)c.re, (double)c.im), make_double2(cs, sn));
+}
+
+// The phased coefficients A0 = a(k) at (row j, column i) and A1 = a(k') at the
+// mirror (jm, im); k' has the same |k| on a 2 pi fftfreq grid (kx(-i) = -kx(i)
+// except at the Nyquist column, where it equals kx(i)), so omega, the phase and
+// its sincos are shared then. Returns |k| at (j, i).
+__device__ __forceinline__ double phased_pair(const SynthArgs& a, int j, int i, int jm, int im,
+                                              double t, double2& A0, double2& A1) {
+  const double x0 = a.kx[i], x1 = a.kx[im], y0 = a.ky[j], y1 = a.ky[jm];
+  const double k0 = sqrt(x0 * x0 + y0 * y0);
+  const double k1 = sqrt(x1 * x1 + y1 * y1);
+  const osar_c32 c0 = a.coefs[(size_t)j * a.nx + i];
+  const osar_c32 c1 = a.coefs[(size_t)jm * a.nx + im];
+  const double om0 = a.depth > 0. ? sqrt(kG * k0 * tanh(k0 * a.depth)) : sqrt(kG * k0);
+  double sn, cs;
+  sincos(-(om0 * t), &sn, &cs);
+  A0 = cmul2(make_double2((double)c0.re, (double)c0.im), make_double2(cs, sn));
+  if (k1 == k0)
+    A1 = cmul2(make_double2((double)c1.re, (double)c1.im), make_double2(cs, sn));
+  else
+    A1 = phased(c1, k1, t, a.depth);
+  return k0;
+}
 
 // Pass 1: one workgroup per row pair (j, -j) and snapshot. Thread t owns the
 // columns i = t, t + kRowThreads, ... of row j together with their mirrors -i of row
-// -j, so a(k) and a(-k) of every packed spectrum it builds are in its own
-// registers; only the FFT buffer lives in LDS (2 rows, 64 KB at 2048).
+// -j, so a(k) and a(-k) of every packed spectrum it builds are its own; only
+// the FFT buffer lives in LDS (2 rows, 64 KB at 2048), so two workgroups share
+// a CU and one's HBM stores overlap the other's FFT.
+// Away from the Nyquist row and column, -k is the true mirror, every
+// multiplier has m(-k) = conj(m(k)) and H(m a) = m H(a): the thread keeps only
+// H(a) = (a(k) + conj(a(-k))) / 2 and 1/|k| per column pair in registers. The
+// Nyquist column's pair (one per row pair) is kept in LDS by its thread; the
+// Nyquist row (one workgroup of ny / 2 + 1), and a table without the mirror
+// symmetry, recompute a(k) and a(-k) for the general form.
 // Rows go to the workspace in column-blocked layout W[q][x / cb][y][x % cb],
 // so that pass 2 reads each block of cb columns as one contiguous run.
-__global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
+__global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double2* S = (double2*)lds_raw;  // [2][nx] packed spectrum of rows j, jm
   double2* tw = S + 2 * a.nx;      // [128] twiddles
+  double2* nyq = tw + 128;         // [2] a(k), a(-k) at the Nyquist column
   load_twiddles(tw, a.twx);
-  const int j = blockIdx.x;
+  // Row pairs in XCD-contiguous order: rows j and j + 1 fill one 128-byte
+  // line of the workspace between them, so their workgroups should run
+  // behind the same L2.
+  const int npairs = a.ny / 2 + 1;
+  const int item = osar::xcd_item(blockIdx.x, npairs * a.n_times);
+  const int j = item % npairs;
   const int jm = (a.ny - j) & (a.ny - 1);
   const int nrow = (j == jm) ? 1 : 2;
-  const int ti = blockIdx.y;
+  const int ti = item / npairs;
   const double t = a.times[ti];
   const int tid = threadIdx.x;
   const int nx = a.nx;
-  const int cb = a.cb;
-  const double kyj = (double)a.ky[j], kyjm = (double)a.ky[jm];
-  double2 A0[kRowPairs], A1[kRowPairs];   // a(k) at (j, i) and a(-k) at (jm, -i)
-  double ik0[kRowPairs], ik1[kRowPairs];  // 1/|k| (0 off the good-k set)
-  double kx0[kRowPairs], kx1[kRowPairs];  // kx at columns i and -i
+  const int cbw = a.cbw;
+  const float kyj = a.ky[j], kyjm = a.ky[jm];
+  double2 H[kRowPairs];    // H(a) at (j, i)
+  double ik0[kRowPairs];   // 1/|k| (0 off the good-k set)
   // Every element of the register arrays is assigned on every path (columns
   // beyond nx are clamped and their results never stored), so the arrays stay
   // in registers.
@@ -355,93 +407,122 @@ __global__ __launch_bounds__(kRowThreads) void synth_rows_kernel(SynthArgs a) {
   for (int u = 0; u < kRowPairs; ++u) {
     const int i = min(tid + u * kRowThreads, nx - 1);
     const int im = (nx - i) & (nx - 1);
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const int row = side == 0 ? j : jm, col = side == 0 ? i : im;
-      const double kx = (double)a.kx[col], ky = side == 0 ? kyj : kyjm;
-      const double k = sqrt(kx * kx + ky * ky);
-      const double om = a.depth > 0. ? sqrt(kG * k * tanh(k * a.depth)) : sqrt(kG * k);
-      double sn, cs;
-      sincos(-(om * t), &sn, &cs);
-      const osar_c32 c = a.coefs[(size_t)row * nx + col];
-      const double2 v = cmul2(make_double2((double)c.re, (double)c.im), make_double2(cs, sn));
-      const double ik = k > a.thr ? 1. / k : 0.;
-      if (side == 0) {
-        A0[u] = v;
-        ik0[u] = ik;
-        kx0[u] = kx;
-      } else {
-        A1[u] = v;
-        ik1[u] = ik;
-        kx1[u] = kx;
-      }
+    double2 A0, A1;
+    const double k0 = phased_pair(a, j, i, jm, im, t, A0, A1);
+    H[u] = make_double2(0.5 * (A0.x + A1.x), 0.5 * (A0.y - A1.y));
+    ik0[u] = k0 > a.thr ? 1. / k0 : 0.;
+    if (i == nx / 2 && tid + u * kRowThreads < nx) {   // only this thread reads them back
+      nyq[0] = A0;
+      nyq[1] = A1;
     }
   }
   for (int qi = 0; qi < a.nq; ++qi) {
     const QPlan q = plan_at(a, qi);
+    const double ya = mono_y(q.ca, kyj), yb = mono_y(q.cb, kyj);
+    const bool da = (q.ca & kMonoImag) != 0, db = (q.cb & kMonoImag) != 0;
+    // Opaque per iteration: keeps the FFT's address arithmetic inside the loop
+    // instead of holding it in registers across all spectra.
+    int logx = a.logx;
+    asm volatile("" : "+s"(logx));
+    unsigned general = 0;   // column pairs that need the general form
 #pragma unroll
     for (int u = 0; u < kRowPairs; ++u) {
       const bool in = tid + u * kRowThreads < nx;
       const int i = min(tid + u * kRowThreads, nx - 1);
       const int im = (nx - i) & (nx - 1);
-      const double kx = kx0[u], kxm = kx1[u];
-      double2 sk, smk;   // S at k = (j, i) and at -k = (jm, im)
-      if (kxm == -kx && kyjm == -kyj) {
-        // Every multiplier has m(-k) = conj(m(k)) where -k is the true mirror,
-        // so H(m a) = m H(a): one H(a) serves every packed spectrum.
-        const double2 h = make_double2(0.5 * (A0[u].x + A1[u].x), 0.5 * (A0[u].y - A1[u].y));
-        sk = cmul2(packed_mult(q.ma, q.mb, kx, kyj, ik0[u]), h);
-        smk = cmul2(packed_mult(q.ma, q.mb, kxm, kyjm, ik1[u]), make_double2(h.x, -h.y));
-      } else {
-        // a Nyquist row or column is its own mirror: the general form
-        const double2 ha = herm(q.ma, kx, kyj, ik0[u], kxm, kyjm, ik1[u], A0[u], A1[u]);
-        const double2 hb = herm(q.mb, kx, kyj, ik0[u], kxm, kyjm, ik1[u], A0[u], A1[u]);
-        sk = make_double2(ha.x - hb.y, ha.y + hb.x);  // ha + i hb
-        const double2 hc = herm(q.ma, kxm, kyjm, ik1[u], kx, kyj, ik0[u], A1[u], A0[u]);
-        const double2 hd = herm(q.mb, kxm, kyjm, ik1[u], kx, kyj, ik0[u], A1[u], A0[u]);
-        smk = make_double2(hc.x - hd.y, hc.y + hd.x);
-      }
-      if (in) {
-        S[i] = sk;
-        // for a self-paired row this rewrites the value the mirror's thread writes
-        S[(nrow - 1) * nx + im] = smk;
+      // Opaque per spectrum as well: the multipliers derived from k are
+      // recomputed here rather than hoisted (one set per multiplier kind).
+      // kx is re-read from the (cached) table at an index made opaque so the
+      // loads stay in the loop.
+      int iq = i;
+      asm volatile("" : "+v"(iq), "+v"(ik0[u]));
+      const double kx = a.kx[iq], kxm = a.kx[(nx - iq) & (nx - 1)];
+      if (kxm == -kx && (double)kyjm == -(double)kyj) {
+        // the packed multiplier mA + i mB at -k is conj(mA(k)) + i conj(mB(k))
+        const double2 h = H[u];
+        const double fa = mono_x(q.ca, kx, ik0[u]) * ya, fb = mono_x(q.cb, kx, ik0[u]) * yb;
+        const double ar = da ? 0. : fa, ai = da ? fa : 0.;
+        const double br = db ? 0. : fb, bi = db ? fb : 0.;
+        if (in) {
+          S[i] = cmul2(make_double2(ar - bi, ai + br), h);
+          // for a self-paired row this rewrites the value the mirror's thread writes
+          S[(nrow - 1) * nx + im] = cmul2(make_double2(ar + bi, br - ai), make_double2(h.x, -h.y));
+        }
+      } else if (in) {
+        general |= 1u << u;
       }
     }
+    // A Nyquist row or column is its own mirror: the general form, from a(k)
+    // and a(-k) themselves (one column pair of each row pair, or the whole
+    // Nyquist row). Kept out of the unrolled loop above.
+#pragma unroll 1
+    for (int u = 0; general != 0; ++u, general >>= 1) {
+      if (!(general & 1u)) continue;
+      const int i = tid + u * kRowThreads;
+      const int im = (nx - i) & (nx - 1);
+      const double kx = a.kx[i], kxm = a.kx[im], ky = kyj, kym = kyjm;
+      double2 A0, A1;
+      double k0;
+      if (i == nx / 2 && j != jm) {
+        A0 = nyq[0];
+        A1 = nyq[1];
+        k0 = sqrt(kx * kx + ky * ky);
+      } else {
+        k0 = phased_pair(a, j, i, jm, im, t, A0, A1);
+      }
+      const double ik = k0 > a.thr ? 1. / k0 : 0.;
+      const double km = sqrt(kxm * kxm + kym * kym);
+      const double ikm = km > a.thr ? 1. / km : 0.;
+      const double yam = mono_y(q.ca, kym), ybm = mono_y(q.cb, kym);
+      const double2 mak = mono_c(da, mono_x(q.ca, kx, ik) * ya);
+      const double2 mbk = mono_c(db, mono_x(q.cb, kx, ik) * yb);
+      const double2 mam = mono_c(da, mono_x(q.ca, kxm, ikm) * yam);
+      const double2 mbm = mono_c(db, mono_x(q.cb, kxm, ikm) * ybm);
+      const double2 ha = herm(mak, A0, mam, A1);
+      const double2 hb = herm(mbk, A0, mbm, A1);
+      S[i] = make_double2(ha.x - hb.y, ha.y + hb.x);  // ha + i hb at k
+      const double2 hc = herm(mam, A1, mak, A0);
+      const double2 hd = herm(mbm, A1, mbk, A0);
+      S[(nrow - 1) * nx + im] = make_double2(hc.x - hd.y, hc.y + hd.x);
+    }
     __syncthreads();
-    lds_ifft<kRowThreads, 2 * kMaxN / kRowThreads, 3, false>(S, nx, a.logx, nrow, 0, nx, 1, tw);
+    lds_ifft<kRowThreads, 2 * kMaxN / kRowThreads, 3, false>(S, nx, logx, nrow, 0, nx, 1, tw);
     double2* w = a.work + (size_t)(ti * a.nq + qi) * a.ny * nx;
     for (int e = tid; e < nrow * nx; e += kRowThreads) {
-      const int r = e >> a.logx, i = e & (nx - 1);
+      const int r = e >> logx, i = e & (nx - 1);
       const int row = r == 0 ? j : jm;
-      const int xb = i >> a.lcb, c = i & (cb - 1);
-      w[((size_t)xb * a.ny + row) * cb + c] = S[e];
+      const int xb = i >> a.lcbw, c = i & (cbw - 1);
+      w[((size_t)xb * a.ny + row) * cbw + c] = S[e];
     }
     __syncthreads();
   }
 }
 
-// Pass 2: one workgroup per (packed spectrum, block of cb columns, snapshot):
-// the block is one contiguous run of the workspace; FFT along y in LDS, then
+// Pass 2: one workgroup per (block of cb columns, packed spectrum, snapshot),
+// in XCD-contiguous order so that the blocks sharing a workspace run (cbw
+// columns of a row) are read behind the same L2; FFT along y in LDS, then
 // scaled by 1/(Nx Ny) and written as float32 fields, x-major (field[x][y]).
-// Pass 2: one workgroup per (packed spectrum, block of cb columns, snapshot):
-// the block is one contiguous run of the workspace; FFT along y in LDS, then
-// scaled by 1/(Nx Ny) and written as float32 fields, x-major (field[x][y]).
-__global__ __launch_bounds__(kColThreads) void synth_cols_kernel(SynthArgs a) {
+__global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  const int cb = a.cb;
+  const int cb = a.cb, cbw = a.cbw;
   double2* C = (double2*)lds_raw;  // [ny][cb]
   double2* tw = C + (size_t)a.ny * cb;  // [128] twiddles
   load_twiddles(tw, a.twy);
-  const int xb = blockIdx.x;
+  const int nxb = a.nx >> a.lcb;
+  const int item = osar::xcd_item(blockIdx.x, nxb * a.nq * a.n_times);
+  const int xb = item % nxb;
+  const int qi = (item / nxb) % a.nq;
+  const int ti = item / (nxb * a.nq);
   const int x0 = xb * cb;
-  const int qi = blockIdx.y;
-  const int ti = blockIdx.z;
   const int tid = threadIdx.x;
   const int ny = a.ny, nx = a.nx;
-  const double2* w = a.work + (size_t)(ti * a.nq + qi) * ny * nx + (size_t)xb * ny * cb;
-  for (int e = tid; e < ny * cb; e += kColThreads) C[e] = w[e];
+  const double2* w = a.work + (size_t)(ti * a.nq + qi) * ny * nx +
+                     (size_t)(x0 >> a.lcbw) * ny * cbw + (x0 & (cbw - 1));
+  for (int e = tid; e < ny * cb; e += kColThreads)
+    C[e] = w[(size_t)(e >> a.lcb) * cbw + (e & (cb - 1))];
   __syncthreads();
-  lds_ifft<kColThreads, kColElems / kColThreads, 4, true>(C, ny, a.logy, cb, a.lcb, 1, cb, tw);
+  lds_ifft<kColThreads, kColElems / kColThreads, kColRadixLog, true>(C, ny, a.logy, cb, a.lcb, 1,
+                                                                     cb, tw);
   const QPlan q = plan_at(a, qi);
   const double scale = 1.0 / ((double)nx * (double)ny);
   const double t = a.times[ti];
@@ -477,6 +558,26 @@ int ilog2(int n) {
   return (1 << l) == n ? l : -1;
 }
 
+// Monomial code of a multiplier (QPlan::ca / cb).
+int mono_code(int m) {
+  auto c = [](int px, int py, int pk, int imag, int neg) {
+    return px | (py << 2) | (pk << 4) | (imag ? kMonoImag : 0) | (neg ? kMonoNeg : 0);
+  };
+  switch (m) {
+    case M_ONE: return c(0, 0, 0, 0, 0);
+    case M_IKXN: return c(1, 0, 1, 1, 0);
+    case M_IKYN: return c(0, 1, 1, 1, 0);
+    case M_IKX: return c(1, 0, 0, 1, 0);
+    case M_IKY: return c(0, 1, 0, 1, 0);
+    case M_NKX2: return c(2, 0, 0, 0, 1);
+    case M_NKY2: return c(0, 2, 0, 0, 1);
+    case M_NKXKY: return c(1, 1, 0, 0, 1);
+    case M_NKX2KI: return c(2, 0, 1, 0, 1);
+    case M_NKY2KI: return c(0, 2, 1, 0, 1);
+    default: return kMonoNone;
+  }
+}
+
 int make_plan(const osar_surface_desc* d, QPlan* q, int* nq) {
   // Real outputs in pairing order with their multipliers.
   struct Item { int out, m; };
@@ -492,7 +593,7 @@ int make_plan(const osar_surface_desc* d, QPlan* q, int* nq) {
     const int f = kOrder[oi];
     if (!(d->fields & (1 << f))) continue;
     if (d->choppy && (f == 3 || f == 4)) {
-      q[n++] = QPlan{f == 3 ? M_IKX : M_IKY, f == 3 ? M_NKX2KI : M_NKY2KI, f, -1, 1};
+      q[n++] = QPlan{f == 3 ? M_IKX : M_IKY, f == 3 ? M_NKX2KI : M_NKY2KI, f, -1, 1, 0, 0};
       continue;
     }
     items.push_back(Item{f, kMult[f]});
@@ -500,7 +601,11 @@ int make_plan(const osar_surface_desc* d, QPlan* q, int* nq) {
   for (size_t i = 0; i < items.size(); i += 2) {
     const bool two = i + 1 < items.size();
     q[n++] = QPlan{items[i].m, two ? items[i + 1].m : (int)M_NONE, items[i].out,
-                   two ? items[i + 1].out : -1, 0};
+                   two ? items[i + 1].out : -1, 0, 0, 0};
+  }
+  for (int i = 0; i < n; ++i) {
+    q[i].ca = mono_code(q[i].ma);
+    q[i].cb = mono_code(q[i].mb);
   }
   *nq = n;
   return OSAR_OK;
@@ -598,8 +703,10 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   if (cb > d->nx) cb = d->nx;
   a.cb = cb;
   a.lcb = ilog2(cb);
+  a.cbw = cb > kWsBlock ? cb : (kWsBlock < d->nx ? kWsBlock : d->nx);
+  a.lcbw = ilog2(a.cbw);
   hipStream_t s = (hipStream_t)stream;
-  const size_t lds_rows = (2 * (size_t)d->nx + 128) * sizeof(double2);
+  const size_t lds_rows = (2 * (size_t)d->nx + 128 + 2) * sizeof(double2);
   const size_t lds_cols = ((size_t)cb * d->ny + 128) * sizeof(double2);
   {
     const int rc1 = osar_reserve_lds((const void*)synth_rows_kernel, lds_rows);
@@ -607,10 +714,11 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
     const int rc2 = osar_reserve_lds((const void*)synth_cols_kernel, lds_cols);
     if (rc2) return rc2;
   }
-  hipLaunchKernelGGL(synth_rows_kernel, dim3(d->ny / 2 + 1, d->n_times), dim3(kRowThreads),
+  a.n_times = d->n_times;
+  hipLaunchKernelGGL(synth_rows_kernel, dim3((d->ny / 2 + 1) * d->n_times), dim3(kRowThreads),
                      lds_rows, s, a);
   OSAR_LAUNCHED("synth_rows_kernel");
-  hipLaunchKernelGGL(synth_cols_kernel, dim3(d->nx / cb, a.nq, d->n_times), dim3(kColThreads),
+  hipLaunchKernelGGL(synth_cols_kernel, dim3((d->nx / cb) * a.nq * d->n_times), dim3(kColThreads),
                      lds_cols, s, a);
   OSAR_LAUNCHED("synth_cols_kernel");
   return OSAR_OK;
