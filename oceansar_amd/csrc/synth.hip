@@ -104,6 +104,23 @@ __device__ __forceinline__ float* out_at(const SynthArgs& a, int i) {
   }
 }
 
+// Streaming (nontemporal) store and load of one complex FP64 value: the
+// workspace and the output fields are written once and read once by the next
+// kernel, so they should not displace cached data (register spill slots, the
+// coefficient rows, the twiddle tables).
+typedef double nt_double2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_stream(double2* p, double2 v) {
+  nt_double2 x;
+  x.x = v.x;
+  x.y = v.y;
+  __builtin_nontemporal_store(x, (nt_double2*)p);
+}
+
+__device__ __forceinline__ double2 load_stream(const double2* p) {
+  const nt_double2 x = __builtin_nontemporal_load((const nt_double2*)p);
+  return make_double2(x.x, x.y);
+}
+
 __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -218,6 +235,12 @@ __device__ __forceinline__ void dft(double2 (&x)[R]) {
   }
 }
 
+// FFT buffers in LDS hold element e at slot pad(e) = e + e / 8: the strided
+// accesses of a Stockham pass (stride R elements of 16 bytes) then fall into
+// distinct banks instead of a few.
+__device__ __forceinline__ int pad(int e) { return e + (e >> 3); }
+__host__ __device__ constexpr size_t padded(size_t n) { return n + (n >> 3); }
+
 // Twiddle exp(+2 pi i k / N) from the two-level LDS table: tw[k & 63] for the
 // low six bits times tw[64 + (k >> 6)] for the rest (one FP64 complex product,
 // within 2 ulp of the directly rounded value).
@@ -255,7 +278,7 @@ __device__ __forceinline__ void stockham_pass(double2* buf, int logN, int Ns, in
       const int s = SEQ_FAST ? (id & (nseq - 1)) : (id >> lnb);
       const int j = SEQ_FAST ? (id >> lseq) : (id & (nb - 1));
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = buf[s * ss + (j + r * nb) * es];
+      for (int r = 0; r < R; ++r) v[b][r] = buf[pad(s * ss + (j + r * nb) * es)];
     }
   }
   __syncthreads();
@@ -279,14 +302,14 @@ __device__ __forceinline__ void stockham_pass(double2* buf, int logN, int Ns, in
       dft<R>(v[b]);
       const int o = (j - jm) * R + jm;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[s * ss + (o + r * Ns) * es] = v[b][r];
+      for (int r = 0; r < R; ++r) buf[pad(s * ss + (o + r * Ns) * es)] = v[b][r];
     }
   }
   __syncthreads();
 }
 
 // In-LDS inverse FFT of nseq power-of-two sequences, element e of sequence s
-// at buf[s * ss + e * es]; tw is the two-level twiddle table in LDS; results
+// at buf[pad(s * ss + e * es)]; tw is the two-level twiddle table in LDS; results
 // stay in buf. Passes take radix 2^min(left, LRMAX) (one step smaller when that
 // would leave a radix-2 pass). EMAX bounds nseq * N / NT, the elements one
 // thread holds in a pass. SEQ_FAST: consecutive threads take consecutive
@@ -294,7 +317,7 @@ __device__ __forceinline__ void stockham_pass(double2* buf, int logN, int Ns, in
 // (es = nseq, ss = 1) where that keeps every LDS access of a wave contiguous;
 // otherwise consecutive threads take consecutive butterflies of one sequence.
 template <int NT, int EMAX, int LRMAX, bool SEQ_FAST>
-__device__ void lds_ifft(double2* buf, int N, int logN, int nseq, int lseq, int ss, int es,
+__device__ __forceinline__ void lds_ifft(double2* buf, int N, int logN, int nseq, int lseq, int ss, int es,
                          const double2* tw) {
   int Ns = 1, logNs = 0, left = logN;
   while (left > 0) {
@@ -381,7 +404,7 @@ __device__ __forceinline__ double phased_pair(const SynthArgs& a, int j, int i, 
 __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double2* S = (double2*)lds_raw;  // [2][nx] packed spectrum of rows j, jm
-  double2* tw = S + 2 * a.nx;      // [128] twiddles
+  double2* tw = S + padded(2 * a.nx);   // [128] twiddles
   double2* nyq = tw + 128;         // [2] a(k), a(-k) at the Nyquist column
   load_twiddles(tw, a.twx);
   // Row pairs in XCD-contiguous order: rows j and j + 1 fill one 128-byte
@@ -399,7 +422,6 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
   const int cbw = a.cbw;
   const float kyj = a.ky[j], kyjm = a.ky[jm];
   double2 H[kRowPairs];    // H(a) at (j, i)
-  double ik0[kRowPairs];   // 1/|k| (0 off the good-k set)
   // Every element of the register arrays is assigned on every path (columns
   // beyond nx are clamped and their results never stored), so the arrays stay
   // in registers.
@@ -408,9 +430,8 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
     const int i = min(tid + u * kRowThreads, nx - 1);
     const int im = (nx - i) & (nx - 1);
     double2 A0, A1;
-    const double k0 = phased_pair(a, j, i, jm, im, t, A0, A1);
+    phased_pair(a, j, i, jm, im, t, A0, A1);
     H[u] = make_double2(0.5 * (A0.x + A1.x), 0.5 * (A0.y - A1.y));
-    ik0[u] = k0 > a.thr ? 1. / k0 : 0.;
     if (i == nx / 2 && tid + u * kRowThreads < nx) {   // only this thread reads them back
       nyq[0] = A0;
       nyq[1] = A1;
@@ -420,37 +441,46 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
     const QPlan q = plan_at(a, qi);
     const double ya = mono_y(q.ca, kyj), yb = mono_y(q.cb, kyj);
     const bool da = (q.ca & kMonoImag) != 0, db = (q.cb & kMonoImag) != 0;
-    // Opaque per iteration: keeps the FFT's address arithmetic inside the loop
-    // instead of holding it in registers across all spectra.
-    int logx = a.logx;
-    asm volatile("" : "+s"(logx));
+    const bool need_ik = ((q.ca | q.cb) & 16) != 0;   // a multiplier divides by |k|
+    // Opaque per iteration: the index and address arithmetic of the build,
+    // the FFT and the write-out is recomputed per spectrum instead of being
+    // held in registers across all of them.
+    int logx = a.logx, tq = tid;
+    asm volatile("" : "+s"(logx), "+v"(tq));
     unsigned general = 0;   // column pairs that need the general form
 #pragma unroll
     for (int u = 0; u < kRowPairs; ++u) {
-      const bool in = tid + u * kRowThreads < nx;
-      const int i = min(tid + u * kRowThreads, nx - 1);
+      const bool in = tq + u * kRowThreads < nx;
+      const int i = min(tq + u * kRowThreads, nx - 1);
       const int im = (nx - i) & (nx - 1);
-      // Opaque per spectrum as well: the multipliers derived from k are
-      // recomputed here rather than hoisted (one set per multiplier kind).
-      // kx is re-read from the (cached) table at an index made opaque so the
-      // loads stay in the loop.
-      int iq = i;
-      asm volatile("" : "+v"(iq), "+v"(ik0[u]));
-      const double kx = a.kx[iq], kxm = a.kx[(nx - iq) & (nx - 1)];
+      // kx is re-read from its (cached) table at the opaque index, so the
+      // multipliers derived from it are recomputed here rather than hoisted
+      // (one set per multiplier kind).
+      const double kx = a.kx[i], kxm = a.kx[im];
       if (kxm == -kx && (double)kyjm == -(double)kyj) {
         // the packed multiplier mA + i mB at -k is conj(mA(k)) + i conj(mB(k))
         const double2 h = H[u];
-        const double fa = mono_x(q.ca, kx, ik0[u]) * ya, fb = mono_x(q.cb, kx, ik0[u]) * yb;
+        // 1/|k| (0 off the good-k set), recomputed rather than kept: registers
+        double ik = 0.;
+        if (need_ik) {
+          const double k = sqrt(kx * kx + (double)kyj * (double)kyj);
+          ik = k > a.thr ? 1. / k : 0.;
+        }
+        const double fa = mono_x(q.ca, kx, ik) * ya, fb = mono_x(q.cb, kx, ik) * yb;
         const double ar = da ? 0. : fa, ai = da ? fa : 0.;
         const double br = db ? 0. : fb, bi = db ? fb : 0.;
         if (in) {
-          S[i] = cmul2(make_double2(ar - bi, ai + br), h);
+          S[pad(i)] = cmul2(make_double2(ar - bi, ai + br), h);
           // for a self-paired row this rewrites the value the mirror's thread writes
-          S[(nrow - 1) * nx + im] = cmul2(make_double2(ar + bi, br - ai), make_double2(h.x, -h.y));
+          S[pad((nrow - 1) * nx + im)] =
+              cmul2(make_double2(ar + bi, br - ai), make_double2(h.x, -h.y));
         }
       } else if (in) {
         general |= 1u << u;
       }
+      // one column pair at a time: interleaving all of them would need more
+      // than the 128 registers two workgroups per CU leave
+      __builtin_amdgcn_sched_barrier(0);
     }
     // A Nyquist row or column is its own mirror: the general form, from a(k)
     // and a(-k) themselves (one column pair of each row pair, or the whole
@@ -458,7 +488,7 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
 #pragma unroll 1
     for (int u = 0; general != 0; ++u, general >>= 1) {
       if (!(general & 1u)) continue;
-      const int i = tid + u * kRowThreads;
+      const int i = tq + u * kRowThreads;
       const int im = (nx - i) & (nx - 1);
       const double kx = a.kx[i], kxm = a.kx[im], ky = kyj, kym = kyjm;
       double2 A0, A1;
@@ -480,19 +510,19 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
       const double2 mbm = mono_c(db, mono_x(q.cb, kxm, ikm) * ybm);
       const double2 ha = herm(mak, A0, mam, A1);
       const double2 hb = herm(mbk, A0, mbm, A1);
-      S[i] = make_double2(ha.x - hb.y, ha.y + hb.x);  // ha + i hb at k
+      S[pad(i)] = make_double2(ha.x - hb.y, ha.y + hb.x);  // ha + i hb at k
       const double2 hc = herm(mam, A1, mak, A0);
       const double2 hd = herm(mbm, A1, mbk, A0);
-      S[(nrow - 1) * nx + im] = make_double2(hc.x - hd.y, hc.y + hd.x);
+      S[pad((nrow - 1) * nx + im)] = make_double2(hc.x - hd.y, hc.y + hd.x);
     }
     __syncthreads();
     lds_ifft<kRowThreads, 2 * kMaxN / kRowThreads, 3, false>(S, nx, logx, nrow, 0, nx, 1, tw);
     double2* w = a.work + (size_t)(ti * a.nq + qi) * a.ny * nx;
-    for (int e = tid; e < nrow * nx; e += kRowThreads) {
+    for (int e = tq; e < nrow * nx; e += kRowThreads) {
       const int r = e >> logx, i = e & (nx - 1);
       const int row = r == 0 ? j : jm;
       const int xb = i >> a.lcbw, c = i & (cbw - 1);
-      w[((size_t)xb * a.ny + row) * cbw + c] = S[e];
+      store_stream(&w[((size_t)xb * a.ny + row) * cbw + c], S[pad(e)]);
     }
     __syncthreads();
   }
@@ -505,8 +535,8 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
 __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int cb = a.cb, cbw = a.cbw;
-  double2* C = (double2*)lds_raw;  // [ny][cb]
-  double2* tw = C + (size_t)a.ny * cb;  // [128] twiddles
+  double2* C = (double2*)lds_raw;  // [cb][ny]
+  double2* tw = C + padded((size_t)a.ny * cb);  // [128] twiddles
   load_twiddles(tw, a.twy);
   const int nxb = a.nx >> a.lcb;
   const int item = osar::xcd_item(blockIdx.x, nxb * a.nq * a.n_times);
@@ -518,11 +548,13 @@ __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(Synt
   const int ny = a.ny, nx = a.nx;
   const double2* w = a.work + (size_t)(ti * a.nq + qi) * ny * nx +
                      (size_t)(x0 >> a.lcbw) * ny * cbw + (x0 & (cbw - 1));
-  for (int e = tid; e < ny * cb; e += kColThreads)
-    C[e] = w[(size_t)(e >> a.lcb) * cbw + (e & (cb - 1))];
+  for (int e = tid; e < ny * cb; e += kColThreads) {
+    const int y = e >> a.lcb, c = e & (cb - 1);
+    C[pad(c * ny + y)] = load_stream(&w[(size_t)y * cbw + c]);
+  }
   __syncthreads();
-  lds_ifft<kColThreads, kColElems / kColThreads, kColRadixLog, true>(C, ny, a.logy, cb, a.lcb, 1,
-                                                                     cb, tw);
+  lds_ifft<kColThreads, kColElems / kColThreads, kColRadixLog, false>(C, ny, a.logy, cb, 0, ny,
+                                                                      1, tw);
   const QPlan q = plan_at(a, qi);
   const double scale = 1.0 / ((double)nx * (double)ny);
   const double t = a.times[ti];
@@ -531,24 +563,24 @@ __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(Synt
   float* const outb = q.outb >= 0 ? out_at(a, q.outb) : nullptr;
   for (int e = tid; e < ny * cb; e += kColThreads) {
     const int c = e >> a.logy, y = e & (ny - 1);  // consecutive threads walk y: contiguous stores
-    const double2 z = C[y * cb + c];
+    const double2 z = C[pad(e)];
     const double re = z.x * scale, im = z.y * scale;
     const size_t o = (size_t)ti * plane + (size_t)(x0 + c) * ny + y;
     if (q.choppy) {
-      outa[o] = (float)(re / (1. + im));
+      __builtin_nontemporal_store((float)(re / (1. + im)), &outa[o]);
       continue;
     }
     if (q.outa >= 0) {
       double v = re;
       if (q.outa == 0) v = v + a.cur_x * t;
       if (q.outa == 1) v = v + a.cur_y * t;
-      outa[o] = (float)v;
+      __builtin_nontemporal_store((float)v, &outa[o]);
     }
     if (q.outb >= 0) {
       double v = im;
       if (q.outb == 0) v = v + a.cur_x * t;
       if (q.outb == 1) v = v + a.cur_y * t;
-      outb[o] = (float)v;
+      __builtin_nontemporal_store((float)v, &outb[o]);
     }
   }
 }
@@ -706,8 +738,8 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   a.cbw = cb > kWsBlock ? cb : (kWsBlock < d->nx ? kWsBlock : d->nx);
   a.lcbw = ilog2(a.cbw);
   hipStream_t s = (hipStream_t)stream;
-  const size_t lds_rows = (2 * (size_t)d->nx + 128 + 2) * sizeof(double2);
-  const size_t lds_cols = ((size_t)cb * d->ny + 128) * sizeof(double2);
+  const size_t lds_rows = (padded(2 * (size_t)d->nx) + 128 + 2) * sizeof(double2);
+  const size_t lds_cols = (padded((size_t)cb * d->ny) + 128) * sizeof(double2);
   {
     const int rc1 = osar_reserve_lds((const void*)synth_rows_kernel, lds_rows);
     if (rc1) return rc1;
