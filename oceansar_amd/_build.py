@@ -1,14 +1,16 @@
-"""Build liboceansar_hip.so (gfx950) in-tree with hipcc."""
+"""Build liboceansar_hip.so (gfx950) in-tree with hipcc: one object per source,
+compiled in parallel, then linked."""
 import os
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'liboceansar_hip.so')
 SOURCES = ['ctx.hip', 'echo.hip', 'bragg.hip', 'synth.hip']
-FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
+FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
          '-Wno-unused-command-line-argument']
 
@@ -16,12 +18,25 @@ FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',
 def build(verbose=False):
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    tmp = LIB + '.tmp'
-    cmd = [hipcc] + FLAGS + ['-I', os.path.join(ROOT, 'include'), '-I', CSRC] + srcs + ['-o', tmp]
-    if verbose:
-        print(' '.join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    inc = ['-I', os.path.join(ROOT, 'include'), '-I', CSRC]
+    with tempfile.TemporaryDirectory(prefix='osar_build_') as tmpdir:
+        objs, procs = [], []
+        for s in srcs:
+            o = os.path.join(tmpdir, os.path.basename(s) + '.o')
+            cmd = [hipcc] + FLAGS + inc + ['-c', s, '-o', o]
+            if verbose:
+                print(' '.join(cmd), flush=True)
+            procs.append((cmd, subprocess.Popen(cmd)))
+            objs.append(o)
+        for cmd, p in procs:
+            if p.wait() != 0:
+                raise subprocess.CalledProcessError(p.returncode, cmd)
+        tmp = LIB + '.tmp'
+        cmd = [hipcc, '--offload-arch=gfx950', '-shared', '-fPIC'] + objs + ['-o', tmp]
+        if verbose:
+            print(' '.join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, LIB)
     return LIB
 
 

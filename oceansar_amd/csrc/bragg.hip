@@ -35,6 +35,7 @@ struct BraggConst {
   double cos_wd, sin_wd;         // wind direction
   double sq_re, sq_im;           // sqrt(epsilon_sw), principal branch
   double dxdy;                   // float32(dx) * float32(dy) as the reference multiplies
+  double inv_cK_p;               // 1 / cK_p
 };
 
 BraggConst bragg_const(const osar_bragg_desc& d) {
@@ -57,25 +58,21 @@ BraggConst bragg_const(const osar_bragg_desc& d) {
   c.sq_re = std::sqrt((eps_abs + 73.) / 2.);
   c.sq_im = 18. / (2. * c.sq_re);
   c.dxdy = (double)(float)d.dx * (double)(float)d.dy;
+  c.inv_cK_p = 1. / c.cK_p;
   return c;
 }
 
 // x^2.5 as x^2 sqrt(x) (the same value as pow(x, 2.5) to rounding)
 __device__ __forceinline__ double pow25(double x) { return x * x * sqrt(x); }
 
-// (cos, sin) of atan(t): (1, t) / sqrt(1 + t^2), written so that |t| -> inf
-// gives (0, sign t) as atan(+-inf) = +-pi/2 does.
+// (cos, sin) of atan(t): (1, t) / sqrt(1 + t^2). Beyond |t| = 1e150 (where
+// t^2 would overflow) cos < 1e-150 is returned as 0 and sin as sign t, the
+// limits atan(+-inf) = +-pi/2 has.
 __device__ __forceinline__ void cos_sin_atan(double t, double* c, double* s) {
-  if (fabs(t) <= 1.) {
-    const double r = rsqrt(1. + t * t);
-    *c = r;
-    *s = t * r;
-  } else {
-    const double u = 1. / t;                       // |u| < 1, 0 for t = +-inf
-    const double r = rsqrt(1. + u * u);
-    *c = fabs(u) * r;
-    *s = copysign(r, t);
-  }
+  const double r = rsqrt(1. + t * t);
+  const bool huge = fabs(t) >= 1e150;
+  *c = huge ? 0. : r;
+  *s = huge ? copysign(1., t) : t * r;
 }
 
 // a / (d d) for complex a, d as a conj(d d) / |d d|^2: one real division
@@ -119,6 +116,7 @@ __device__ double interp_fill1(double x, const double* xp, const double* fp, int
 }
 
 constexpr int kThreads = 256;
+constexpr int kMaxColsPerBlock = 256;   // columns one workgroup walks (1 / cos_inc in LDS)
 
 // Every angle of romeiser97.py:262-314 enters only
 // through its sine and cosine, so atan/asin/sin/cos are replaced by the exact
@@ -131,7 +129,8 @@ constexpr int kThreads = 256;
 // its RCS sums to rcs_sum[i][pol] (one atomic per workgroup and pol). Each
 // workgroup grid-strides over the facets of its refresh.
 template <bool UNIFORM>
-__global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d, BraggConst c) {
+__global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d, BraggConst c,
+                                                               int cols_per_block) {
   const size_t n = (size_t)d.nx * d.ny;
   const int npol = d.pol == 2 ? 2 : 1;
   const int ri = blockIdx.y;
@@ -142,21 +141,36 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
   float* amp_s = d.amp + (size_t)ri * d.amp_stride;
   const double az_now = (pulse * d.t_step - d.t_span / 2.) * d.v_ground;
   double sums[2] = {0., 0.};
-  for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < n;
-       idx += (size_t)gridDim.x * kThreads) {
-    // fields are x-major: idx = col * ny + row
-    const int col = (int)(idx / d.ny);
-    const int row = (int)(idx - (size_t)col * d.ny);
-    // azimuth angle az = asin(arg) of the row (sar_raw_nompi.py:363)
-    const double arg = (d.y_row[row] - az_now) / d.gr0;
-    const double sin_az = fabs(arg) <= 1. ? arg : NAN;
-    const double cos_az = sqrt(1. - arg * arg);
+  // Thread = one facet row; the workgroup walks a range of columns of 256
+  // consecutive rows (fields are x-major, idx = col * ny + row, so every
+  // column is one contiguous read per wave). The row's azimuth terms are
+  // computed once: az = asin(arg) (sar_raw_nompi.py:363) as (sin az, cos az),
+  // and A = az - wind_dir as (cos A, sin A).
+  const int nrb = (d.ny + kThreads - 1) / kThreads;
+  const int row = (int)(blockIdx.x % nrb) * kThreads + threadIdx.x;
+  const int cs = (int)(blockIdx.x / nrb);
+  const int col_lo = cs * cols_per_block;
+  const int col_hi = row < d.ny ? min(d.nx, col_lo + cols_per_block) : col_lo;
+  const double arg = row < d.ny ? (d.y_row[row] - az_now) / d.gr0 : 0.;
+  const double sin_az = fabs(arg) <= 1. ? arg : NAN;
+  const double cos_az = sqrt(1. - arg * arg);
+  const double cos_A = cos_az * c.cos_wd + sin_az * c.sin_wd;
+  const double sin_A = sin_az * c.cos_wd - cos_az * c.sin_wd;
+  __shared__ double sec_inc[kMaxColsPerBlock];   // 1 / cos(inc) of the workgroup's columns
+  for (int i = threadIdx.x; i < min(d.nx - col_lo, cols_per_block); i += kThreads)
+    sec_inc[i] = 1. / d.cos_inc[col_lo + i];
+  __syncthreads();
+  for (int col = col_lo; col < col_hi; ++col) {
+    const size_t idx = (size_t)col * d.ny + row;
     const double cos_inc = d.cos_inc[col], sin_inc = d.sin_inc[col];
     const double diffx = (double)diffx_s[idx];
     const double diffy = (double)diffy_s[idx];
-    // parallel and orthogonal slopes s_p = atan(u), s_t = atan(v)
-    double cos_sp, sin_sp, cos_st, sin_st;
-    cos_sin_atan(cos_az * diffx + sin_az * diffy, &cos_sp, &sin_sp);
+    // parallel and orthogonal slopes s_p = atan(u), s_t = atan(v) (|u|, |v| are
+    // below 1e39: float32 slopes), with 1 / cos(s_p) = (1 + u^2) cos(s_p)
+    const double u = cos_az * diffx + sin_az * diffy;
+    const double r_sp = rsqrt(1. + u * u);
+    const double cos_sp = r_sp, sin_sp = u * r_sp, sec_sp = (1. + u * u) * r_sp;
+    double cos_st, sin_st;
     cos_sin_atan(-sin_az * diffx + cos_az * diffy, &cos_st, &sin_st);
     const double cos_s_t_2 = cos_st * cos_st;
     const double sin_s_t_2 = 1 - cos_s_t_2;
@@ -170,7 +184,7 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
     const cd sq = {c.sq_re, c.sq_im};
     const cd b_hh = cdiv_sq(eps, cadd({cos_ti, 0.}, sq));
     const cd b_vv = cdiv_sq(cscale(cmulc(eps, eps), 1. + sin_ti_2), cadd(cscale(eps, cos_ti), sq));
-    const double wgt = cos_ip / (cos_inc * cos_sp);
+    const double wgt = cos_ip * (sec_inc[col - col_lo] * sec_sp);   // cos_ip / (cos_inc cos_sp)
     const double inv_ti = 1. / sin_ti_2;
     const double a = sin_s_t_2 * inv_ti;
     const double b = sin_ip_2 * cos_s_t_2 * inv_ti;
@@ -183,18 +197,17 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
     // phi_b = az + atan(q); th0 = phi_b - wind_dir; cos(th0) from angle sums
     double cos_q, sin_q;
     cos_sin_atan((cos_ip * -sin_st) / sin_ip, &cos_q, &sin_q);
-    const double cos_A = cos_az * c.cos_wd + sin_az * c.sin_wd;  // A = az - wind_dir
-    const double sin_A = sin_az * c.cos_wd - cos_az * c.sin_wd;
     const double cth0 = cos_A * cos_q - sin_A * sin_q;
     const bool bad = 4 * sin_ip_2 < d.d * d.d;
     const double S1 = interp_fill1<UNIFORM>(k_b, d.spec_k, d.spec_S, d.n_spec, d.spec_k0,
                                             d.spec_dk, d.spec_inv_dk);
+    const double inv_kb = 1. / k_b;
     double delta = 0.;
     if (d.spread == 0) {
-      const double cK = sqrt(kG / k_b + kSurfT / kRho * k_b);
-      delta = tanh(c.a_0 + 4. * pow25(cK / c.cK_p) + c.a_m * pow25(c.cK_m / cK));
+      const double cK = sqrt(kG * inv_kb + kSurfT / kRho * k_b);
+      delta = tanh(c.a_0 + 4. * pow25(cK * c.inv_cK_p) + c.a_m * pow25(c.cK_m / cK));
     }
-    const double base = S1 / k_b;
+    const double base = S1 * inv_kb;
     for (int comp = 0; comp < 2; ++comp) {
       const double cth = comp == 0 ? cth0 : -cth0;
       double g;
@@ -210,7 +223,9 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
         // romeiser97.py:366-392 (DP: hh masks only the plus component)
         const bool mask = bad && !(d.pol == 2 && is_hh && comp == 0);
         if (mask) rcs = 0.;
-        amp_s[((size_t)ip * 2 + comp) * n + idx] = (float)sqrt(rcs);
+        // float32 amplitude: the square root taken in float32 (within an ulp
+        // of rounding the FP64 root)
+        amp_s[((size_t)ip * 2 + comp) * n + idx] = sqrtf((float)rcs);
         sums[ip] += rcs;
       }
     }
@@ -273,13 +288,21 @@ extern "C" int osar_facet_bragg(osar_ctx* ctx, const osar_bragg_desc* d, void* s
   const size_t want = (n + kThreads - 1) / kThreads;
   if (per > want) per = want;
   const BraggConst c = bragg_const(*d);
-  const dim3 grid((unsigned)per, (unsigned)n_ref);
+  // workgroups per refresh: row blocks x column ranges, about 8 per CU in all
+  const size_t nrb = ((size_t)d->ny + kThreads - 1) / kThreads;
+  size_t ncs = (per + nrb - 1) / nrb;
+  if (ncs > (size_t)d->nx) ncs = d->nx;
+  const size_t ncs_min = ((size_t)d->nx + kMaxColsPerBlock - 1) / kMaxColsPerBlock;
+  if (ncs < ncs_min) ncs = ncs_min;
+  const int cpb = (int)(((size_t)d->nx + ncs - 1) / ncs);
+  ncs = ((size_t)d->nx + cpb - 1) / cpb;
+  const dim3 grid((unsigned)(nrb * ncs), (unsigned)n_ref);
   if (d->spec_uniform)
-    hipLaunchKernelGGL(facet_bragg_kernel<true>, grid, dim3(kThreads), 0, (hipStream_t)stream,
-                       *d, c);
+    hipLaunchKernelGGL(facet_bragg_kernel<true>, grid, dim3(kThreads), 0, (hipStream_t)stream, *d,
+                       c, cpb);
   else
     hipLaunchKernelGGL(facet_bragg_kernel<false>, grid, dim3(kThreads), 0, (hipStream_t)stream,
-                       *d, c);
+                       *d, c, cpb);
   OSAR_LAUNCHED("facet_bragg_kernel");
   return OSAR_OK;
 }
