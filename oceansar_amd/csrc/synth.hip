@@ -197,13 +197,18 @@ __device__ __forceinline__ void dft(double2 (&x)[R]) {
   } else if constexpr (R == 4) {
     dft4(x[0], x[1], x[2], x[3]);
   } else if constexpr (R == 8) {
-    // r = r1 + 4 r2, k = 2 k1 + k2: radix-2 over r2, twiddle w8^(r1 k2), radix-4 over r1
+    // r = r1 + 4 r2, k = 2 k1 + k2: radix-2 over r2, twiddle w8^r1 on the odd
+    // half (1, (1 + i) h, i, (-1 + i) h with h = 1/sqrt 2), radix-4 over r1
+    constexpr double h = 0.70710678118654752440;
     double2 a0[4], a1[4];
 #pragma unroll
-    for (int r1 = 0; r1 < 4; ++r1) {
-      a0[r1] = cadd2(x[r1], x[r1 + 4]);
-      a1[r1] = cmul2(csub2(x[r1], x[r1 + 4]), w16(2 * r1));
-    }
+    for (int r1 = 0; r1 < 4; ++r1) a0[r1] = cadd2(x[r1], x[r1 + 4]);
+    a1[0] = csub2(x[0], x[4]);
+    const double2 d1 = csub2(x[1], x[5]);
+    a1[1] = make_double2(h * (d1.x - d1.y), h * (d1.x + d1.y));
+    a1[2] = cmul_i(csub2(x[2], x[6]));
+    const double2 d3 = csub2(x[3], x[7]);
+    a1[3] = make_double2(-h * (d3.x + d3.y), h * (d3.x - d3.y));
     dft4(a0[0], a0[1], a0[2], a0[3]);
     dft4(a1[0], a1[1], a1[2], a1[3]);
 #pragma unroll
@@ -255,30 +260,44 @@ __device__ __forceinline__ void load_twiddles(double2* dst, const double2* src) 
   for (int i = threadIdx.x; i < 128; i += blockDim.x) dst[i] = src[i];
 }
 
-// One radix-R pass of the inverse Stockham FFT (Ns = product of the radices
-// of the passes before). Butterfly j of a sequence reads x_r = e[j + r N/R],
-// multiplies by w^(r jm N/(Ns R)) (jm = j mod Ns, w = exp(2 pi i / N)), takes
-// the radix-R DFT and writes y_r to e[(j - jm) R + jm + r Ns]. Every thread
-// loads all of its butterflies before the barrier, so the pass runs in place;
-// at most BPT butterflies per thread (checked by the caller's sizes).
-template <int NT, int R, int BPT, bool SEQ_FAST>
-__device__ __forceinline__ void stockham_pass(double2* buf, int logN, int Ns, int logNs,
-                                              int nseq, int lseq, int ss, int es,
-                                              const double2* tw) {
+// One radix-R pass of the inverse Stockham FFT over nseq sequences of length N
+// (Ns = product of the radices of the passes before); element k of sequence s
+// is at buf[pad(s * ss + k)], ss a multiple of 8. Butterfly j of a sequence
+// reads x_r = e[j + r N/R], multiplies by w^(r jm N/(Ns R)) (jm = j mod Ns,
+// w = exp(2 pi i / N); all 1 in the first pass), takes the radix-R DFT and
+// writes y_r to e[(j - jm) R + jm + r Ns]. Every thread loads all of its
+// butterflies before the barrier, so the pass runs in place; at most BPT
+// butterflies per thread (checked by the caller's sizes). Padded addresses:
+// pad(b + r m) = pad(b) + r (m + m / 8) when m is a multiple of 8, and
+// pad(b + r) = pad(b) + r when b mod 8 + r < 8 (the first pass's writes).
+// SHORT: every pass of the transform has N/R and Ns (after the first) multiples
+// of 8, which holds for N >= 64 with radix 8/4 passes; otherwise (N = 16, 32)
+// every address is padded on its own.
+template <int NT, int R, int BPT, bool SHORT>
+__device__ __forceinline__ void stockham_pass(double2* buf, int logN, int Ns, int logNs, int nseq,
+                                              int ss, const double2* tw) {
   constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
   const int tid = threadIdx.x;
   const int lnb = logN - LR;
   const int nb = 1 << lnb;
   const int total = nseq << lnb;
+  const bool rd_fast = SHORT || (nb & 7) == 0;
+  const int rd_step = nb + (nb >> 3);
   double2 v[BPT][R];
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     const int id = tid + b * NT;
     if (id < total) {
-      const int s = SEQ_FAST ? (id & (nseq - 1)) : (id >> lnb);
-      const int j = SEQ_FAST ? (id >> lseq) : (id & (nb - 1));
+      const int s = id >> lnb, j = id & (nb - 1);
+      const int base = s * ss + j;
+      if (rd_fast) {
+        const double2* p = buf + pad(base);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = buf[pad(s * ss + (j + r * nb) * es)];
+        for (int r = 0; r < R; ++r) v[b][r] = p[r * rd_step];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = buf[pad(base + r * nb)];
+      }
     }
   }
   __syncthreads();
@@ -286,61 +305,71 @@ __device__ __forceinline__ void stockham_pass(double2* buf, int logN, int Ns, in
   for (int b = 0; b < BPT; ++b) {
     const int id = tid + b * NT;
     if (id < total) {
-      const int s = SEQ_FAST ? (id & (nseq - 1)) : (id >> lnb);
-      const int j = SEQ_FAST ? (id >> lseq) : (id & (nb - 1));
+      const int s = id >> lnb, j = id & (nb - 1);
       const int jm = j & (Ns - 1);
-      const int step = jm << (logN - logNs - LR);   // jm * N / (Ns * R)
-      // w^(r step) by repeated multiplication with w^step (R - 2 products; the
-      // rounding error grows by about one ulp per product, far below float32)
-      const double2 w1 = twiddle(tw, step);
-      double2 wr = w1;
+      if (Ns > 1) {
+        // w^(r step) by repeated multiplication with w^step (R - 2 products; the
+        // rounding error grows by about one ulp per product, far below float32)
+        const int step = jm << (logN - logNs - LR);   // jm * N / (Ns * R)
+        const double2 w1 = twiddle(tw, step);
+        double2 wr = w1;
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        v[b][r] = cmul2(v[b][r], wr);
-        if (r + 1 < R) wr = cmul2(wr, w1);
+        for (int r = 1; r < R; ++r) {
+          v[b][r] = cmul2(v[b][r], wr);
+          if (r + 1 < R) wr = cmul2(wr, w1);
+        }
       }
       dft<R>(v[b]);
-      const int o = (j - jm) * R + jm;
+      const int o = s * ss + (j - jm) * R + jm;
+      double2* p = buf + pad(o);
+      if (Ns == 1) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[pad(s * ss + (o + r * Ns) * es)] = v[b][r];
+        for (int r = 0; r < R; ++r) p[r] = v[b][r];
+      } else if (SHORT || (Ns & 7) == 0) {
+        const int st = Ns + (Ns >> 3);
+#pragma unroll
+        for (int r = 0; r < R; ++r) p[r * st] = v[b][r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[pad(o + r * Ns)] = v[b][r];
+      }
     }
   }
   __syncthreads();
 }
 
-// In-LDS inverse FFT of nseq power-of-two sequences, element e of sequence s
-// at buf[pad(s * ss + e * es)]; tw is the two-level twiddle table in LDS; results
-// stay in buf. Passes take radix 2^min(left, LRMAX) (one step smaller when that
-// would leave a radix-2 pass). EMAX bounds nseq * N / NT, the elements one
-// thread holds in a pass. SEQ_FAST: consecutive threads take consecutive
-// sequences (nseq a power of two, log2 lseq), for interleaved layouts
-// (es = nseq, ss = 1) where that keeps every LDS access of a wave contiguous;
-// otherwise consecutive threads take consecutive butterflies of one sequence.
-template <int NT, int EMAX, int LRMAX, bool SEQ_FAST>
-__device__ __forceinline__ void lds_ifft(double2* buf, int N, int logN, int nseq, int lseq, int ss, int es,
-                         const double2* tw) {
+// In-LDS inverse FFT of nseq power-of-two sequences of length N, element k of
+// sequence s at buf[pad(s * ss + k)]; tw is the two-level twiddle table in LDS;
+// results stay in buf. Passes take radix 2^min(left, LRMAX) (one step smaller
+// when that would leave a radix-2 pass). EMAX bounds nseq * N / NT, the
+// elements one thread holds in a pass. Consecutive threads take consecutive
+// butterflies of one sequence. SHORT (N >= 64): see stockham_pass.
+template <int NT, int EMAX, int LRMAX, bool SHORT>
+__device__ __forceinline__ void lds_ifft(double2* buf, int N, int logN, int nseq, int ss,
+                                         const double2* tw) {
+  (void)N;
   int Ns = 1, logNs = 0, left = logN;
   while (left > 0) {
     int lr = left < LRMAX ? left : LRMAX;
     if (left - lr == 1 && lr > 2) --lr;
     switch (lr) {
       case 1:
-        stockham_pass<NT, 2, (EMAX / 2 > 0 ? EMAX / 2 : 1), SEQ_FAST>(buf, logN, Ns, logNs, nseq,
-                                                                      lseq, ss, es, tw);
+        stockham_pass<NT, 2, (EMAX / 2 > 0 ? EMAX / 2 : 1), SHORT>(buf, logN, Ns, logNs, nseq, ss,
+                                                                  tw);
         break;
       case 2:
-        stockham_pass<NT, 4, (EMAX / 4 > 0 ? EMAX / 4 : 1), SEQ_FAST>(buf, logN, Ns, logNs, nseq,
-                                                                      lseq, ss, es, tw);
+        stockham_pass<NT, 4, (EMAX / 4 > 0 ? EMAX / 4 : 1), SHORT>(buf, logN, Ns, logNs, nseq, ss,
+                                                                  tw);
         break;
       case 3:
         if constexpr (LRMAX >= 3)
-          stockham_pass<NT, 8, (EMAX / 8 > 0 ? EMAX / 8 : 1), SEQ_FAST>(buf, logN, Ns, logNs,
-                                                                        nseq, lseq, ss, es, tw);
+          stockham_pass<NT, 8, (EMAX / 8 > 0 ? EMAX / 8 : 1), SHORT>(buf, logN, Ns, logNs, nseq,
+                                                                    ss, tw);
         break;
       default:
         if constexpr (LRMAX >= 4)
-          stockham_pass<NT, 16, (EMAX / 16 > 0 ? EMAX / 16 : 1), SEQ_FAST>(buf, logN, Ns, logNs,
-                                                                           nseq, lseq, ss, es, tw);
+          stockham_pass<NT, 16, (EMAX / 16 > 0 ? EMAX / 16 : 1), SHORT>(buf, logN, Ns, logNs, nseq,
+                                                                        ss, tw);
         break;
     }
     Ns <<= lr;
@@ -401,6 +430,7 @@ __device__ __forceinline__ double phased_pair(const SynthArgs& a, int j, int i, 
 // symmetry, recompute a(k) and a(-k) for the general form.
 // Rows go to the workspace in column-blocked layout W[q][x / cb][y][x % cb],
 // so that pass 2 reads each block of cb columns as one contiguous run.
+template <bool SHORT>
 __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double2* S = (double2*)lds_raw;  // [2][nx] packed spectrum of rows j, jm
@@ -516,7 +546,7 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
       S[pad((nrow - 1) * nx + im)] = make_double2(hc.x - hd.y, hc.y + hd.x);
     }
     __syncthreads();
-    lds_ifft<kRowThreads, 2 * kMaxN / kRowThreads, 3, false>(S, nx, logx, nrow, 0, nx, 1, tw);
+    lds_ifft<kRowThreads, 2 * kMaxN / kRowThreads, 3, SHORT>(S, nx, logx, nrow, nx, tw);
     double2* w = a.work + (size_t)(ti * a.nq + qi) * a.ny * nx;
     for (int e = tq; e < nrow * nx; e += kRowThreads) {
       const int r = e >> logx, i = e & (nx - 1);
@@ -532,6 +562,7 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
 // in XCD-contiguous order so that the blocks sharing a workspace run (cbw
 // columns of a row) are read behind the same L2; FFT along y in LDS, then
 // scaled by 1/(Nx Ny) and written as float32 fields, x-major (field[x][y]).
+template <bool SHORT>
 __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   const int cb = a.cb, cbw = a.cbw;
@@ -553,8 +584,7 @@ __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(Synt
     C[pad(c * ny + y)] = load_stream(&w[(size_t)y * cbw + c]);
   }
   __syncthreads();
-  lds_ifft<kColThreads, kColElems / kColThreads, kColRadixLog, false>(C, ny, a.logy, cb, 0, ny,
-                                                                      1, tw);
+  lds_ifft<kColThreads, kColElems / kColThreads, kColRadixLog, SHORT>(C, ny, a.logy, cb, ny, tw);
   const QPlan q = plan_at(a, qi);
   const double scale = 1.0 / ((double)nx * (double)ny);
   const double t = a.times[ti];
@@ -740,18 +770,29 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   hipStream_t s = (hipStream_t)stream;
   const size_t lds_rows = (padded(2 * (size_t)d->nx) + 128 + 2) * sizeof(double2);
   const size_t lds_cols = (padded((size_t)cb * d->ny) + 128) * sizeof(double2);
+  // short padded addressing for transforms of 64 points or more
+  const bool short_x = d->nx >= 64, short_y = d->ny >= 64;
+  const void* rows_k = short_x ? (const void*)synth_rows_kernel<true>
+                               : (const void*)synth_rows_kernel<false>;
+  const void* cols_k = short_y ? (const void*)synth_cols_kernel<true>
+                               : (const void*)synth_cols_kernel<false>;
   {
-    const int rc1 = osar_reserve_lds((const void*)synth_rows_kernel, lds_rows);
+    const int rc1 = osar_reserve_lds(rows_k, lds_rows);
     if (rc1) return rc1;
-    const int rc2 = osar_reserve_lds((const void*)synth_cols_kernel, lds_cols);
+    const int rc2 = osar_reserve_lds(cols_k, lds_cols);
     if (rc2) return rc2;
   }
   a.n_times = d->n_times;
-  hipLaunchKernelGGL(synth_rows_kernel, dim3((d->ny / 2 + 1) * d->n_times), dim3(kRowThreads),
-                     lds_rows, s, a);
+  const dim3 grid_rows((d->ny / 2 + 1) * d->n_times), grid_cols((d->nx / cb) * a.nq * d->n_times);
+  if (short_x)
+    hipLaunchKernelGGL(synth_rows_kernel<true>, grid_rows, dim3(kRowThreads), lds_rows, s, a);
+  else
+    hipLaunchKernelGGL(synth_rows_kernel<false>, grid_rows, dim3(kRowThreads), lds_rows, s, a);
   OSAR_LAUNCHED("synth_rows_kernel");
-  hipLaunchKernelGGL(synth_cols_kernel, dim3((d->nx / cb) * a.nq * d->n_times), dim3(kColThreads),
-                     lds_cols, s, a);
+  if (short_y)
+    hipLaunchKernelGGL(synth_cols_kernel<true>, grid_cols, dim3(kColThreads), lds_cols, s, a);
+  else
+    hipLaunchKernelGGL(synth_cols_kernel<false>, grid_cols, dim3(kColThreads), lds_cols, s, a);
   OSAR_LAUNCHED("synth_cols_kernel");
   return OSAR_OK;
 }

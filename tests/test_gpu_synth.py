@@ -72,6 +72,19 @@ def test_synth_matches_oracle_rectangular_choppy(need_gpu, tmp_path_factory):
     _synth_compare(sc, [0.0, 1.7], ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy'], ('D', 'Diff'))
 
 
+@pytest.mark.parametrize('lx,ly', [(16., 32.), (32., 16.)])
+def test_synth_matches_oracle_16_point_transforms(need_gpu, tmp_path_factory, lx, ly):
+    """16-point row or column transforms: the padded-address fallback of the
+    LDS FFT (longer transforms take the short addressing)."""
+    from oceansar_amd import configs
+    text = configs.workload('small', L=lx, dx=1.)
+    text = text.replace('Ly=%.1f' % lx, 'Ly=%.1f' % ly)
+    sc = configs.load_scene(text, tmp_path_factory.mktemp('s16'))
+    assert (sc.Nx, sc.Ny) == (int(lx), int(ly))
+    names = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
+    _synth_compare(sc, [0.0, 2.5], names, ('D', 'Diff', 'Diff2'))
+
+
 def test_end_to_end_matches_oracle(need_gpu, tmp_path_factory):
     """Full GPU loop (synthesis, Bragg, echo) on a pulse window crossing chunks."""
     import torch
