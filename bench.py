@@ -184,7 +184,10 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--workload', default='large', choices=configs_names())
     ap.add_argument('--pulses', type=int, default=None, help='shorter scene (testing only)')
-    ap.add_argument('--chunk', type=int, default=32, help='snapshot intervals per launch group')
+    ap.add_argument('--chunk', type=int, default=128,
+                    help='snapshot intervals per launch group (128: about 41 GB of fields, '
+                         'workspace and amplitudes on the device for the 2048^2 workload; '
+                         '32 / 64 / 256 measured 3 / 1 / -1 %% slower / slower / faster)')
     ap.add_argument('--cpu-sample', type=int, default=96)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--overlap', action='store_true',
