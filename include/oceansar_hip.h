@@ -210,7 +210,10 @@ typedef struct osar_surface_desc {
   int32_t fields;           /* OSAR_F_* mask of outputs to write */
   int32_t choppy;           /* 1: choppy slope / curvature corrections */
   int32_t n_times;          /* snapshots computed by this call */
-  int32_t reserved;
+  int32_t mirror_grid;      /* 1: kx, ky are a 2 pi fftfreq grid (kx[0] = 0 and
+                               kx[nx - i] = -kx[i] for i != nx / 2, the same for
+                               ky): selects the faster row pass; 0 gives the
+                               same results for any tables */
   double depth;             /* water depth, <= 0 for deep water */
   double current_x, current_y;
   double kx_res, ky_res;    /* float32 grid steps (good-k threshold) */

@@ -55,7 +55,8 @@ class BraggDesc(ctypes.Structure):
 
 
 class SurfaceDesc(ctypes.Structure):
-    _fields_ = [(n, c_int32) for n in ('nx', 'ny', 'fields', 'choppy', 'n_times', 'reserved')] + \
+    _fields_ = [(n, c_int32) for n in ('nx', 'ny', 'fields', 'choppy', 'n_times',
+                                       'mirror_grid')] + \
         [(n, c_double) for n in ('depth', 'current_x', 'current_y', 'kx_res', 'ky_res')] + \
         [(n, c_void_p) for n in ('kx', 'ky', 'coefs', 'times')] + [('out', c_void_p * 8)]
 

@@ -424,13 +424,18 @@ __device__ __forceinline__ double phased_pair(const SynthArgs& a, int j, int i, 
 // a CU and one's HBM stores overlap the other's FFT.
 // Away from the Nyquist row and column, -k is the true mirror, every
 // multiplier has m(-k) = conj(m(k)) and H(m a) = m H(a): the thread keeps only
-// H(a) = (a(k) + conj(a(-k))) / 2 and 1/|k| per column pair in registers. The
-// Nyquist column's pair (one per row pair) is kept in LDS by its thread; the
-// Nyquist row (one workgroup of ny / 2 + 1), and a table without the mirror
-// symmetry, recompute a(k) and a(-k) for the general form.
+// H(a) = (a(k) + conj(a(-k))) / 2 per column pair in registers. The Nyquist
+// column's pair (one per row pair) is kept in LDS by its thread.
+// MIRROR (the tables are a 2 pi fftfreq grid: kx(-i) = -kx(i) off the Nyquist
+// column, kx(0) = 0, the same for ky): the only elements that need the general
+// form are the Nyquist column and the self-paired rows 0 and ny / 2, and a
+// self-paired row keeps its own a(k) in the second half of the FFT buffer (a
+// single row is transformed), so no a(k) is recomputed inside the spectrum
+// loop and H(a) stays in registers. Otherwise the general form recomputes a(k)
+// and a(-k) where it needs them.
 // Rows go to the workspace in column-blocked layout W[q][x / cb][y][x % cb],
 // so that pass 2 reads each block of cb columns as one contiguous run.
-template <bool SHORT>
+template <bool SHORT, bool MIRROR>
 __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double2* S = (double2*)lds_raw;  // [2][nx] packed spectrum of rows j, jm
@@ -466,7 +471,9 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
       nyq[0] = A0;
       nyq[1] = A1;
     }
+    if (MIRROR && j == jm && tid + u * kRowThreads < nx) S[pad(nx + i)] = A0;
   }
+  if (MIRROR) __syncthreads();   // a self-paired row reads the coefficients of the mirror columns
   for (int qi = 0; qi < a.nq; ++qi) {
     const QPlan q = plan_at(a, qi);
     const double ya = mono_y(q.ca, kyj), yb = mono_y(q.cb, kyj);
@@ -526,6 +533,11 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
       if (i == nx / 2 && j != jm) {
         A0 = nyq[0];
         A1 = nyq[1];
+        k0 = sqrt(kx * kx + ky * ky);
+      } else if (MIRROR) {
+        // self-paired row: a(-k) is this row's coefficient at column im
+        A0 = S[pad(nx + i)];
+        A1 = S[pad(nx + im)];
         k0 = sqrt(kx * kx + ky * ky);
       } else {
         k0 = phased_pair(a, j, i, jm, im, t, A0, A1);
@@ -772,8 +784,11 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   const size_t lds_cols = (padded((size_t)cb * d->ny) + 128) * sizeof(double2);
   // short padded addressing for transforms of 64 points or more
   const bool short_x = d->nx >= 64, short_y = d->ny >= 64;
-  const void* rows_k = short_x ? (const void*)synth_rows_kernel<true>
-                               : (const void*)synth_rows_kernel<false>;
+  const bool mirror = d->mirror_grid != 0;
+  const void* rows_k = short_x ? (mirror ? (const void*)synth_rows_kernel<true, true>
+                                         : (const void*)synth_rows_kernel<true, false>)
+                               : (mirror ? (const void*)synth_rows_kernel<false, true>
+                                         : (const void*)synth_rows_kernel<false, false>);
   const void* cols_k = short_y ? (const void*)synth_cols_kernel<true>
                                : (const void*)synth_cols_kernel<false>;
   {
@@ -784,10 +799,18 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   }
   a.n_times = d->n_times;
   const dim3 grid_rows((d->ny / 2 + 1) * d->n_times), grid_cols((d->nx / cb) * a.nq * d->n_times);
-  if (short_x)
-    hipLaunchKernelGGL(synth_rows_kernel<true>, grid_rows, dim3(kRowThreads), lds_rows, s, a);
+  if (short_x && mirror)
+    hipLaunchKernelGGL((synth_rows_kernel<true, true>), grid_rows, dim3(kRowThreads), lds_rows, s,
+                       a);
+  else if (short_x)
+    hipLaunchKernelGGL((synth_rows_kernel<true, false>), grid_rows, dim3(kRowThreads), lds_rows,
+                       s, a);
+  else if (mirror)
+    hipLaunchKernelGGL((synth_rows_kernel<false, true>), grid_rows, dim3(kRowThreads), lds_rows,
+                       s, a);
   else
-    hipLaunchKernelGGL(synth_rows_kernel<false>, grid_rows, dim3(kRowThreads), lds_rows, s, a);
+    hipLaunchKernelGGL((synth_rows_kernel<false, false>), grid_rows, dim3(kRowThreads), lds_rows,
+                       s, a);
   OSAR_LAUNCHED("synth_rows_kernel");
   if (short_y)
     hipLaunchKernelGGL(synth_cols_kernel<true>, grid_cols, dim3(kColThreads), lds_cols, s, a);

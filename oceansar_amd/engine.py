@@ -21,7 +21,18 @@ def _t(a, dtype, device):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
 
 
-def synth_fields(ctx, surf, kx, ky, coefs, times, out, fields=None):
+def mirror_grid(k):
+    """True when the float32 wave numbers k form a 2 pi fftfreq grid as the
+    synthesis row pass uses it: k[0] = 0 and k[n - i] = -k[i] for i != n / 2."""
+    k = np.asarray(k, dtype=np.float32)
+    n = len(k)
+    i = np.arange(n)
+    ok = k[(n - i) % n] == -k[i]
+    ok[n // 2] = True
+    return bool(ok.all())
+
+
+def synth_fields(ctx, surf, kx, ky, coefs, times, out, fields=None, mirror=None):
     """osar_surface_synth: fields of surface `surf` (a SurfaceInit) at `times`.
 
     kx, ky, coefs: device tensors (float32 [nx], [ny], complex64 [ny][nx]).
@@ -48,6 +59,9 @@ def synth_fields(ctx, surf, kx, ky, coefs, times, out, fields=None):
     d.fields = mask
     d.choppy = 1 if surf.choppy else 0
     d.n_times = len(times)
+    if mirror is None:   # checked on the host copies of the uploaded tables
+        mirror = mirror_grid(surf.kx_s) and mirror_grid(surf.ky_s)
+    d.mirror_grid = 1 if mirror else 0
     d.depth = -1.0 if surf.depth is None else float(surf.depth)
     d.current_x, d.current_y = float(surf.current[0]), float(surf.current[1])
     d.kx_res, d.ky_res = float(surf.kx_res), float(surf.ky_res)

@@ -85,6 +85,30 @@ def test_synth_matches_oracle_16_point_transforms(need_gpu, tmp_path_factory, lx
     _synth_compare(sc, [0.0, 2.5], names, ('D', 'Diff', 'Diff2'))
 
 
+@pytest.mark.parametrize('lx', [16., 256.])
+def test_synth_general_row_pass_matches_mirror_pass(need_gpu, tmp_path_factory, lx):
+    """The row pass for arbitrary wave-number tables (mirror_grid = 0: a(k) of
+    the Nyquist row and column recomputed where needed) and the one for the
+    2 pi fftfreq grid (kept in LDS) give the same bits."""
+    import torch
+    from oceansar_amd import configs
+    from oceansar_amd.engine import DeviceScene, synth_fields
+    sc = configs.load_scene(configs.workload('small', L=lx, dx=1.), tmp_path_factory.mktemp('sg'))
+    ds = DeviceScene(sc)
+    names = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
+    res = []
+    for mirror in (True, False):
+        out = {n: torch.empty((2, sc.Nx, sc.Ny), dtype=torch.float32, device='cuda')
+               for n in names}
+        keep = synth_fields(ds.ctx, sc.surface, ds.kx, ds.ky, ds.coefs, [0.7, 3.1], out, names,
+                            mirror=mirror)
+        torch.cuda.synchronize()
+        del keep
+        res.append({n: out[n].cpu().numpy() for n in names})
+    for n in names:
+        assert np.array_equal(res[0][n], res[1][n]), n
+
+
 def test_end_to_end_matches_oracle(need_gpu, tmp_path_factory):
     """Full GPU loop (synthesis, Bragg, echo) on a pulse window crossing chunks."""
     import torch
