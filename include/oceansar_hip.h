@@ -173,6 +173,9 @@ typedef struct osar_bragg_desc {
   const double* y_row;      /* [ny] */
   const double* spec_k;     /* [n_spec] ascending */
   const double* spec_S;     /* [n_spec] */
+  const double* spec_slope; /* [n_spec] (spec_S[i] - spec_S[i-1]) / (spec_k[i] -
+                               spec_k[i-1]) for i >= 1, or NULL (computed per
+                               facet; the same value) */
   const float* diffx;       /* [nx][ny] (x-major) */
   const float* diffy;
   float* amp;               /* out: [n_pol][2][nx][ny] sqrt(RCS) per refresh */

@@ -47,8 +47,8 @@ class BraggDesc(ctypes.Structure):
                                  't_step', 't_span', 'v_ground', 'gr0')] + \
         [(n, c_int32) for n in ('spec_uniform', 'reserved')] + \
         [(n, c_double) for n in ('spec_k0', 'spec_dk', 'spec_inv_dk')] + \
-        [(n, c_void_p) for n in ('cos_inc', 'sin_inc', 'y_row', 'spec_k', 'spec_S', 'diffx',
-                                 'diffy', 'amp', 'rcs_sum')] + \
+        [(n, c_void_p) for n in ('cos_inc', 'sin_inc', 'y_row', 'spec_k', 'spec_S', 'spec_slope',
+                                 'diffx', 'diffy', 'amp', 'rcs_sum')] + \
         [(n, c_int32) for n in ('n_refresh', 'snap0')] + \
         [(n, c_void_p) for n in ('refresh_pulse', 'refresh_snap')] + \
         [(n, ctypes.c_int64) for n in ('diff_stride', 'amp_stride')]

@@ -98,10 +98,11 @@ __device__ __forceinline__ double table_x(const double* xp, double x0, double dk
 
 // scipy interp1d linear with fill_value=1. The first index with xp[i] >= x
 // (searchsorted side='left') is found from a guess on the table's mean step and
-// corrected by a local walk, so any ascending table gives the exact index.
+// corrected by a local walk, so any ascending table gives the exact index. The
+// interval slope comes from the precomputed table when there is one.
 template <bool UNIFORM>
-__device__ double interp_fill1(double x, const double* xp, const double* fp, int n, double x0,
-                               double dk, double inv_dk) {
+__device__ double interp_fill1(double x, const double* xp, const double* fp, const double* sl,
+                               int n, double x0, double dk, double inv_dk) {
   if (x < table_x<UNIFORM>(xp, x0, dk, 0) || x > table_x<UNIFORM>(xp, x0, dk, n - 1)) return 1.0;
   if (x != x) return x;  // NaN propagates, as in scipy
   int i = (int)fmin(fmax(ceil((x - x0) * inv_dk), 0.), (double)(n - 1));
@@ -109,9 +110,9 @@ __device__ double interp_fill1(double x, const double* xp, const double* fp, int
   while (i > 0 && table_x<UNIFORM>(xp, x0, dk, i - 1) >= x) --i;
   const int idx = i < 1 ? 1 : i;
   const double xl = table_x<UNIFORM>(xp, x0, dk, idx - 1);
-  const double xh = table_x<UNIFORM>(xp, x0, dk, idx);
-  const double yl = fp[idx - 1], yh = fp[idx];
-  const double slope = (yh - yl) / (xh - xl);
+  const double yl = fp[idx - 1];
+  const double slope =
+      sl ? sl[idx] : (fp[idx] - yl) / (table_x<UNIFORM>(xp, x0, dk, idx) - xl);
   return slope * (x - xl) + yl;
 }
 
@@ -199,8 +200,8 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
     cos_sin_atan((cos_ip * -sin_st) / sin_ip, &cos_q, &sin_q);
     const double cth0 = cos_A * cos_q - sin_A * sin_q;
     const bool bad = 4 * sin_ip_2 < d.d * d.d;
-    const double S1 = interp_fill1<UNIFORM>(k_b, d.spec_k, d.spec_S, d.n_spec, d.spec_k0,
-                                            d.spec_dk, d.spec_inv_dk);
+    const double S1 = interp_fill1<UNIFORM>(k_b, d.spec_k, d.spec_S, d.spec_slope, d.n_spec,
+                                            d.spec_k0, d.spec_dk, d.spec_inv_dk);
     const double inv_kb = 1. / k_b;
     double delta = 0.;
     if (d.spread == 0) {

@@ -160,6 +160,12 @@ class DeviceScene:
             self.spec_k = _t(sc.bragg.spec_k, f64, dev)
             self.spec_grid = arange_grid(sc.bragg.spec_k)
             self.spec_S = _t(sc.bragg.spec_S, f64, dev)
+            # per-interval slopes of the linear interpolation, the same FP64
+            # quotient the kernel would take per facet
+            k, S = np.asarray(sc.bragg.spec_k, np.float64), np.asarray(sc.bragg.spec_S, np.float64)
+            slope = np.zeros_like(S)
+            slope[1:] = (S[1:] - S[:-1]) / (k[1:] - k[:-1])
+            self.spec_slope = _t(slope, f64, dev)
         surf = sc.surface
         self.coefs = _t(surf.wave_coefs, c64, dev)
         self.kx = _t(surf.kx_s, f32, dev)
@@ -294,6 +300,7 @@ class DeviceScene:
         d.cos_inc, d.sin_inc = _lib.ptr(self.cos_inc), _lib.ptr(self.sin_inc)
         d.y_row = _lib.ptr(self.y_row)
         d.spec_k, d.spec_S = _lib.ptr(self.spec_k), _lib.ptr(self.spec_S)
+        d.spec_slope = _lib.ptr(self.spec_slope)
         d.spec_k0, d.spec_dk, d.spec_uniform = self.spec_grid
         k = b.spec_k
         d.spec_inv_dk = float((len(k) - 1) / (k[-1] - k[0])) if len(k) > 1 else 0.
