@@ -200,13 +200,33 @@ struct FacetIdx {
   int m, p;
 };
 
-__device__ __forceinline__ void load_index(const osar_echo_desc& d, size_t idx, bool bragg,
+// Element at a 32-bit byte offset from a wave-uniform base: the load takes the
+// base in scalar registers and only the offset in a vector register, so loads
+// that share an offset share its arithmetic.
+template <typename T>
+__device__ __forceinline__ T load_at(const T* base, uint32_t byte_off) {
+  return *(const T*)((const char*)base + byte_off);
+}
+
+__device__ __forceinline__ void load_index(const osar_echo_desc& d, uint32_t idx4, bool bragg,
                                            FacetIdx& fi) {
   if (bragg) {
-    fi.m = d.ind0_m[idx];
-    fi.p = d.ind0_p[idx];
+    fi.m = load_at(d.ind0_m, idx4);
+    fi.p = load_at(d.ind0_p, idx4);
   }
 }
+
+// Wave-uniform bases of the per-facet inputs of a workgroup's pulse group: the
+// snapshot fields at the group's first snapshot (a lane adds its own snapshot
+// offset), and per lane the amplitude plane of its RCS refresh.
+template <int NPOL>
+struct FacetBases {
+  const float* dx;
+  const float* dy;
+  const float* dz;
+  const float* s[5];
+  const float* amp[NPOL][2];   // per lane
+};
 
 // Per-lane inputs of one facet (one column x, the lane's row and pulse),
 // including the column's geometry (the same for every lane): all of it is
@@ -220,10 +240,14 @@ struct FacetIn {
   float s0[5], s1[5];  // specular slope / curvature fields at m and m+1
 };
 
+// idx4: byte offset of the facet in a field plane; f0, f1: the lane's byte
+// offsets of its snapshots m and m + 1 (f1 = f0 at a snapshot pulse, where the
+// second value is not used) from the group's first snapshot.
 template <int NPOL, bool SPEC, bool XTI>
-__device__ __forceinline__ void load_facet(const osar_echo_desc& d, int x, size_t idx, size_t so,
-                                           size_t plane, int rk, int p, int rslot, bool bragg,
-                                           const FacetIdx& fi, FacetIn<NPOL, SPEC>& f) {
+__device__ __forceinline__ void load_facet(const osar_echo_desc& d, const FacetBases<NPOL>& B,
+                                           int x, uint32_t idx4, uint32_t f0, uint32_t f1, int p,
+                                           bool bragg, const FacetIdx& fi,
+                                           FacetIn<NPOL, SPEC>& f) {
   // the column x is the same for every lane of the wave: read the column
   // tables through the scalar cache into scalar registers
   f.sr = uniform_load(d.sr_col, x);
@@ -232,30 +256,27 @@ __device__ __forceinline__ void load_facet(const osar_echo_desc& d, int x, size_
   if (bragg) f.cap = uniform_load(d.cap_rate, x);
   if (SPEC) f.inc = uniform_load(d.inc_col, x);
   if (XTI) f.dinc = uniform_load(d.dinc_col, x);
-  f.dx0 = d.dx[so];
-  f.dy0 = d.dy[so];
-  f.dz0 = d.dz[so];
-  if (rk != 0) {
-    f.dx1 = d.dx[so + plane];
-    f.dy1 = d.dy[so + plane];
-    f.dz1 = d.dz[so + plane];
-  }
+  const uint32_t o0 = idx4 + f0, o1 = idx4 + f1;
+  f.dx0 = load_at(B.dx, o0);
+  f.dy0 = load_at(B.dy, o0);
+  f.dz0 = load_at(B.dz, o0);
+  f.dx1 = load_at(B.dx, o1);
+  f.dy1 = load_at(B.dy, o1);
+  f.dz1 = load_at(B.dz, o1);
   if (bragg) {
 #pragma unroll
     for (int pol = 0; pol < NPOL; ++pol) {
-      const size_t ao = ((size_t)(rslot * NPOL + pol) * 2) * plane + idx;
-      f.amp[pol][0] = d.amp[ao];
-      f.amp[pol][1] = d.amp[ao + plane];
+      f.amp[pol][0] = load_at(B.amp[pol][0], idx4);
+      f.amp[pol][1] = load_at(B.amp[pol][1], idx4);
     }
-    f.sm = d.scat_m[fi.m + p];
-    f.sp = d.scat_p[fi.p + p];
+    f.sm = d.scat_m[(uint32_t)(fi.m + p)];
+    f.sp = d.scat_p[(uint32_t)(fi.p + p)];
   }
   if (SPEC) {
-    const float* src[5] = {d.sx, d.sy, d.sxx, d.syy, d.sxy};
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      f.s0[k] = src[k][so];
-      if (rk != 0) f.s1[k] = src[k][so + plane];
+      f.s0[k] = load_at(B.s[k], o0);
+      f.s1[k] = load_at(B.s[k], o1);
     }
   }
 }
@@ -402,11 +423,35 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
   const float wh = (float)((double)rk / (double)d.K);   // w_h, w_l rounded to float32
   const float wl = (float)(1.0 - (double)wh);
   const size_t plane = (size_t)d.nx * d.ny;
-  const size_t snap_off = (size_t)(m - d.snap0) * plane;
   const bool bragg = (d.flags & OSAR_ECHO_BRAGG) != 0;
   int rslot = bragg ? d.rcs_slot[pc] - d.rcs0 : 0;
   const bool bragg_ok = bragg && rslot >= 0 && rslot < d.n_rcs_slots;
   if (!bragg_ok) rslot = 0;
+  // Snapshot fields from the group's first snapshot m_lo = pstart / K (every
+  // lane's m is at most P / K + 1 above it; the host bounds the byte offsets to
+  // 32 bits); at a snapshot pulse (rk = 0) the second value repeats the first.
+  FacetBases<NPOL> B;
+  {
+    const size_t fb = (size_t)(pstart / d.K - d.snap0) * plane;
+    B.dx = d.dx + fb;
+    B.dy = d.dy + fb;
+    B.dz = d.dz + fb;
+    if (SPEC) {
+      B.s[0] = d.sx + fb;
+      B.s[1] = d.sy + fb;
+      B.s[2] = d.sxx + fb;
+      B.s[3] = d.syy + fb;
+      B.s[4] = d.sxy + fb;
+    }
+#pragma unroll
+    for (int pol = 0; pol < NPOL; ++pol) {
+      const float* a0 = d.amp + ((size_t)(rslot * NPOL + pol) * 2) * plane;
+      B.amp[pol][0] = a0;
+      B.amp[pol][1] = a0 + plane;
+    }
+  }
+  const uint32_t f0 = (uint32_t)((m - pstart / d.K) * plane * 4);
+  const uint32_t f1 = f0 + (rk != 0 ? (uint32_t)(plane * 4) : 0u);
   Lines L;
   L.acc = acc;
   L.LS = LS;
@@ -451,18 +496,17 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
     // are in flight while column x is processed
     FacetIn<NPOL, SPEC> cur, nxt;
     FacetIdx ia = {0, 0}, ib = {0, 0};
-    const size_t idx_lo = (size_t)x_lo * d.ny + yrow;
-    load_index(d, idx_lo, bragg_ok, ia);
-    load_facet<NPOL, SPEC, XTI>(d, x_lo, idx_lo, snap_off + idx_lo, plane, rk, p, rslot,
-                                bragg_ok, ia, cur);
-    if (x_lo + 1 < x_hi) load_index(d, idx_lo + d.ny, bragg_ok, ia);
+    const uint32_t col4 = (uint32_t)d.ny * 4u;   // bytes between columns of a field plane
+    const uint32_t idx4_lo = ((uint32_t)x_lo * d.ny + yrow) * 4u;
+    load_index(d, idx4_lo, bragg_ok, ia);
+    load_facet<NPOL, SPEC, XTI>(d, B, x_lo, idx4_lo, f0, f1, p, bragg_ok, ia, cur);
+    if (x_lo + 1 < x_hi) load_index(d, idx4_lo + col4, bragg_ok, ia);
     for (int x = x_lo; x < x_hi; ++x) {
-      const size_t idx = (size_t)x * d.ny + yrow;
+      const uint32_t idx4 = idx4_lo + (uint32_t)(x - x_lo) * col4;
       if (x + 1 < x_hi) {
         ib = ia;
-        if (x + 2 < x_hi) load_index(d, idx + 2 * (size_t)d.ny, bragg_ok, ia);
-        load_facet<NPOL, SPEC, XTI>(d, x + 1, idx + d.ny, snap_off + idx + d.ny, plane, rk, p,
-                                    rslot, bragg_ok, ib, nxt);
+        if (x + 2 < x_hi) load_index(d, idx4 + 2 * col4, bragg_ok, ia);
+        load_facet<NPOL, SPEC, XTI>(d, B, x + 1, idx4 + col4, f0, f1, p, bragg_ok, ib, nxt);
       }
       const double Dx = lerp2(cur.dx0, cur.dx1, rk, wl, wh);
       const double Dy = lerp2(cur.dy0, cur.dy1, rk, wl, wh);
@@ -755,6 +799,11 @@ extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void
   int W = d->window_bins ? d->window_bins : 512;
   OSAR_REQUIRE(R >= 1 && R <= 1024 && W >= 8 && W <= 65536 && d->cols_per_block >= 0,
                OSAR_EINVAL, "bad launch shape R=%d W=%d cols=%d", R, W, d->cols_per_block);
+  // field inputs are addressed by 32-bit byte offsets from a pulse group's
+  // first snapshot (its lanes are at most P / K + 2 snapshots further on)
+  OSAR_REQUIRE((uint64_t)d->nx * d->ny * 4 * (uint64_t)(P / d->K + 3) < (1ull << 32),
+               OSAR_EUNSUPPORTED, "surface of %d x %d facets too large for 32-bit field offsets",
+               d->nx, d->ny);
   hipStream_t s = (hipStream_t)stream;
   OSAR_HIP(hipSetDevice(ctx->device));
   switch (P) {
