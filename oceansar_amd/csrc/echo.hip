@@ -229,11 +229,12 @@ struct FacetBases {
 };
 
 // Per-lane inputs of one facet (one column x, the lane's row and pulse),
-// including the column's geometry (the same for every lane): all of it is
-// loaded one column ahead, so no load is waited for in the column that issues it.
+// loaded one column ahead, so no load is waited for in the column that issues
+// it. The column's geometry (the same for every lane) is read in the column's
+// own iteration through the scalar cache: carrying it through the pipeline
+// would cost vector register moves every column.
 template <int NPOL, bool SPEC>
 struct FacetIn {
-  double sr, ci, si, cap, inc, dinc;  // sr_col, cos_inc, sin_inc, cap_rate, inc_col, dinc_col
   float dx0, dy0, dz0, dx1, dy1, dz1;
   float amp[NPOL][2];
   osar_c32 sm, sp;
@@ -248,14 +249,7 @@ __device__ __forceinline__ void load_facet(const osar_echo_desc& d, const FacetB
                                            int x, uint32_t idx4, uint32_t f0, uint32_t f1, int p,
                                            bool bragg, const FacetIdx& fi,
                                            FacetIn<NPOL, SPEC>& f) {
-  // the column x is the same for every lane of the wave: read the column
-  // tables through the scalar cache into scalar registers
-  f.sr = uniform_load(d.sr_col, x);
-  f.ci = uniform_load(d.cos_inc, x);
-  f.si = uniform_load(d.sin_inc, x);
-  if (bragg) f.cap = uniform_load(d.cap_rate, x);
-  if (SPEC) f.inc = uniform_load(d.inc_col, x);
-  if (XTI) f.dinc = uniform_load(d.dinc_col, x);
+  (void)x;
   const uint32_t o0 = idx4 + f0, o1 = idx4 + f1;
   f.dx0 = load_at(B.dx, o0);
   f.dy0 = load_at(B.dy, o0);
@@ -511,7 +505,9 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
       const double Dx = lerp2(cur.dx0, cur.dx1, rk, wl, wh);
       const double Dy = lerp2(cur.dy0, cur.dy1, rk, wl, wh);
       const double Dz = lerp2(cur.dz0, cur.dz1, rk, wl, wh);
-      const Geo g = facet_range(cur.sr, cur.ci, cur.si, mig, sin_az, Dx, Dy,
+      const double c_sr = uniform_load(d.sr_col, x), c_ci = uniform_load(d.cos_inc, x),
+                   c_si = uniform_load(d.sin_inc, x);
+      const Geo g = facet_range(c_sr, c_ci, c_si, mig, sin_az, Dx, Dy,
                                 Dz, d.sr0);
       const double xb = osar::div_by(g.srs - d.min_sr, d.binsize, inv_binsize);
       const bool finite = fabs(xb) < 1.0e9;   // non-finite or absurd range: no valid tap
@@ -524,7 +520,7 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
       for (int pol = 0; pol < NPOL; ++pol) scene[pol] = make_float2(0.f, 0.f);
       if (bragg_ok) {
         const double surf = -two_k0 * g.srs;
-        const double cap = cur.cap * (double)(p + 1);
+        const double cap = uniform_load(d.cap_rate, x) * (double)(p + 1);
         const float2 e0 = osar::cis_fp64(surf - cap);
         const float2 e1 = osar::cis_fp64(surf + cap);
         const float2 b0 = osar::cmul(e0, make_float2(cur.sm.re, cur.sm.im));
@@ -538,7 +534,7 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
       if (SPEC) {
         // Esn = exp(-i 2 k0 sr_surface) (4 pi)^1.5 E   (sar_raw_nompi.py:405-426)
         zd e[2];
-        if (finite && ka_spa(d.k0, spol, cur.inc, phi, lerp2(cur.s0[0], cur.s1[0], rk, wl, wh),
+        if (finite && ka_spa(d.k0, spol, uniform_load(d.inc_col, x), phi, lerp2(cur.s0[0], cur.s1[0], rk, wl, wh),
                              lerp2(cur.s0[1], cur.s1[1], rk, wl, wh),
                              lerp2(cur.s0[2], cur.s1[2], rk, wl, wh),
                              lerp2(cur.s0[3], cur.s1[3], rk, wl, wh),
@@ -608,7 +604,7 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
             if (ch >= nch) break;
             float2 f = fch[ch];
             if (d.b_xti[ch] != 0.0) {
-              const double tot = cur.dinc + g.dinc;
+              const double tot = uniform_load(d.dinc_col, x) + g.dinc;
               f = osar::cmul(f, osar::cis_fp64(-(d.k0 * d.b_xti[ch]) * tot));
             }
 #pragma unroll
