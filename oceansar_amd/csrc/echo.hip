@@ -278,9 +278,11 @@ __device__ __forceinline__ void load_facet(const osar_echo_desc& d, const FacetB
 // Field value between two snapshots as the reference's snapshot balancer
 // interpolates (oceansar/surfaces/balancer.py:274-282): float32 products and
 // sum, w_l * a + w_h * b (no fused multiply-add), widened to FP64 for the range.
+// At a snapshot pulse (rk = 0: w_l = 1, w_h = 0) b is the same snapshot's
+// value (load_facet), and 1 * a + 0 * a is exactly a.
 __device__ __forceinline__ double lerp2(float a, float b, int rk, float wl, float wh) {
 #pragma clang fp contract(off)
-  if (rk == 0) return (double)a;
+  (void)rk;
   const float x = wl * a;
   const float y = wh * b;
   return (double)(x + y);

@@ -591,9 +591,18 @@ __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(Synt
   const int ny = a.ny, nx = a.nx;
   const double2* w = a.work + (size_t)(ti * a.nq + qi) * ny * nx +
                      (size_t)(x0 >> a.lcbw) * ny * cbw + (x0 & (cbw - 1));
-  for (int e = tid; e < ny * cb; e += kColThreads) {
-    const int y = e >> a.lcb, c = e & (cb - 1);
-    C[pad(c * ny + y)] = load_stream(&w[(size_t)y * cbw + c]);
+  // every load of the block in flight before the first LDS store
+  constexpr int kFill = kColElems / kColThreads;
+  double2 fill[kFill];
+#pragma unroll
+  for (int k = 0; k < kFill; ++k) {
+    const int e = tid + k * kColThreads;
+    if (e < ny * cb) fill[k] = load_stream(&w[(size_t)(e >> a.lcb) * cbw + (e & (cb - 1))]);
+  }
+#pragma unroll
+  for (int k = 0; k < kFill; ++k) {
+    const int e = tid + k * kColThreads;
+    if (e < ny * cb) C[pad((e & (cb - 1)) * ny + (e >> a.lcb))] = fill[k];
   }
   __syncthreads();
   lds_ifft<kColThreads, kColElems / kColThreads, kColRadixLog, SHORT>(C, ny, a.logy, cb, ny, tw);
