@@ -109,6 +109,20 @@ def bench_batch(args):
     W = br.contributions
     br.run_sequential()                       # warm-up: workspaces, kernel attributes
     seq = min(br.run_sequential() for _ in range(2))
+    # the plain path per side length (synchronised after each scene): where the
+    # plain path leaves the device underused, batching can gain
+    times = []
+    br.run_sequential(per_scene=times)
+    per_side = {}
+    for sc, dt in zip(scenes, times):
+        c = per_side.setdefault(str(sc.Nx), {'scenes': 0, 'time_s': 0., 'contributions': 0})
+        c['scenes'] += 1
+        c['time_s'] += dt
+        c['contributions'] += sc.contributions
+    tot = sum(times)
+    for c in per_side.values():
+        c['time_share'] = c['time_s'] / tot
+        c['contributions_per_s'] = c['contributions'] / c['time_s']
     conc = min(br.run_concurrent() for _ in range(2))
     t = time.perf_counter()
     br.capture()
@@ -126,6 +140,7 @@ def bench_batch(args):
                    'tuned_classes': len(tuner.cache)},
         'makespan_s': {'plain_sequential': seq, 'concurrent_streams': conc, 'graph_replay': graph},
         'plain_value': W / seq, 'speedup_vs_plain': seq / graph,
+        'plain_per_side': per_side,
         'host_s': {'scene_setup': t_host, 'upload_and_tune': t_setup, 'graph_capture': t_capture},
     }
 

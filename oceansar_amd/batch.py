@@ -267,14 +267,17 @@ class BatchRunner:
         torch.cuda.synchronize()
         return time.perf_counter() - t
 
-    def run_sequential(self):
+    def run_sequential(self, per_scene=None):
         """The plain path: one scene after another on one stream with the
-        device's shared context; returns the makespan in seconds."""
+        device's shared context; returns the makespan in seconds. With a list
+        `per_scene`, the device is synchronised after every scene and each
+        scene's time is appended to it (a separate, slightly slower run)."""
         torch = self.torch
         states = self._starts()
         torch.cuda.synchronize()
         t = time.perf_counter()
         for ds, st in zip(self.ds, states):
+            t_scene = time.perf_counter()
             self._rewind(st)
             self._zero(ds, st)
             ctx = ds.ctx
@@ -284,6 +287,9 @@ class BatchRunner:
                     pass
             finally:
                 ds.ctx = ctx
+            if per_scene is not None:
+                torch.cuda.synchronize()
+                per_scene.append(time.perf_counter() - t_scene)
         torch.cuda.synchronize()
         return time.perf_counter() - t
 
