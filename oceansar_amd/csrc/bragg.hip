@@ -128,12 +128,14 @@ constexpr int kMaxColsPerBlock = 256;   // columns one workgroup walks (1 / cos_
 // Grid (workgroups per refresh, refreshes): refresh i = blockIdx.y reads the
 // slopes of snapshot slot refresh_snap[i], writes amplitude slot i and adds
 // its RCS sums to rcs_sum[i][pol] (one atomic per workgroup and pol). Each
-// workgroup grid-strides over the facets of its refresh.
-template <bool UNIFORM>
+// workgroup grid-strides over the facets of its refresh. POL (0 vv, 1 hh, 2
+// DP) and SPREAD (0 Elfouhaily, 1 cos^2) are compile-time: fewer live values
+// and branches in the facet loop.
+template <bool UNIFORM, int POL, int SPREAD>
 __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d, BraggConst c,
                                                                int cols_per_block) {
   const size_t n = (size_t)d.nx * d.ny;
-  const int npol = d.pol == 2 ? 2 : 1;
+  constexpr int npol = POL == 2 ? 2 : 1;
   const int ri = blockIdx.y;
   const int pulse = d.n_refresh > 0 ? d.refresh_pulse[ri] : d.pulse;
   const size_t snap = d.n_refresh > 0 ? (size_t)(d.refresh_snap[ri] - d.snap0) : 0;
@@ -204,7 +206,7 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
                                             d.spec_k0, d.spec_dk, d.spec_inv_dk);
     const double inv_kb = 1. / k_b;
     double delta = 0.;
-    if (d.spread == 0) {
+    if (SPREAD == 0) {
       const double cK = sqrt(kG * inv_kb + kSurfT / kRho * k_b);
       // tanh(x) = sign(x) (1 - e) / (1 + e), e = exp(-2 |x|): one exponential,
       // no branches, within 3 ulp of the library tanh (x >= ln(2) / 4 here)
@@ -217,16 +219,16 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
       const double cth = comp == 0 ? cth0 : -cth0;
       double g;
       constexpr double kInvPi = 0.31830988618379067154;
-      if (d.spread == 0) g = cth > 0. ? (1. + delta * (2. * cth * cth - 1.)) * kInvPi : 0.;
+      if (SPREAD == 0) g = cth > 0. ? (1. + delta * (2. * cth * cth - 1.)) * kInvPi : 0.;
       else g = cth < 0. ? 0. : kInvPi * cth * cth;
       const double spectrum = base * g;
       // pol 2 (DP): slot 0 = hh, slot 1 = vv
       for (int ip = 0; ip < npol; ++ip) {
-        const bool is_hh = (d.pol == 1) || (d.pol == 2 && ip == 0);
+        const bool is_hh = (POL == 1) || (POL == 2 && ip == 0);
         const double T = is_hh ? T_hh : T_vv;
         double rcs = T * spectrum * c.dxdy;
         // romeiser97.py:366-392 (DP: hh masks only the plus component)
-        const bool mask = bad && !(d.pol == 2 && is_hh && comp == 0);
+        const bool mask = bad && !(POL == 2 && is_hh && comp == 0);
         if (mask) rcs = 0.;
         // float32 amplitude: the square root taken in float32 (within an ulp
         // of rounding the FP64 root)
@@ -252,6 +254,31 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
       atomicAdd(&d.rcs_sum[(size_t)ri * npol + ip], v);
     }
   }
+}
+
+template <bool UNIFORM, int POL>
+void launch_bragg_pol(int spread, dim3 grid, hipStream_t s, const osar_bragg_desc& d,
+                      const BraggConst& c, int cpb) {
+  if (spread == 0)
+    hipLaunchKernelGGL((facet_bragg_kernel<UNIFORM, POL, 0>), grid, dim3(kThreads), 0, s, d, c,
+                       cpb);
+  else
+    hipLaunchKernelGGL((facet_bragg_kernel<UNIFORM, POL, 1>), grid, dim3(kThreads), 0, s, d, c,
+                       cpb);
+}
+
+template <bool UNIFORM>
+void launch_bragg_u(int pol, int spread, dim3 grid, hipStream_t s, const osar_bragg_desc& d,
+                    const BraggConst& c, int cpb) {
+  if (pol == 0) launch_bragg_pol<UNIFORM, 0>(spread, grid, s, d, c, cpb);
+  else if (pol == 1) launch_bragg_pol<UNIFORM, 1>(spread, grid, s, d, c, cpb);
+  else launch_bragg_pol<UNIFORM, 2>(spread, grid, s, d, c, cpb);
+}
+
+void launch_bragg(bool uniform, int pol, int spread, dim3 grid, hipStream_t s,
+                  const osar_bragg_desc& d, const BraggConst& c, int cpb) {
+  if (uniform) launch_bragg_u<true>(pol, spread, grid, s, d, c, cpb);
+  else launch_bragg_u<false>(pol, spread, grid, s, d, c, cpb);
 }
 
 }  // namespace
@@ -302,12 +329,7 @@ extern "C" int osar_facet_bragg(osar_ctx* ctx, const osar_bragg_desc* d, void* s
   const int cpb = (int)(((size_t)d->nx + ncs - 1) / ncs);
   ncs = ((size_t)d->nx + cpb - 1) / cpb;
   const dim3 grid((unsigned)(nrb * ncs), (unsigned)n_ref);
-  if (d->spec_uniform)
-    hipLaunchKernelGGL(facet_bragg_kernel<true>, grid, dim3(kThreads), 0, (hipStream_t)stream, *d,
-                       c, cpb);
-  else
-    hipLaunchKernelGGL(facet_bragg_kernel<false>, grid, dim3(kThreads), 0, (hipStream_t)stream,
-                       *d, c, cpb);
+  launch_bragg(d->spec_uniform != 0, d->pol, d->spread, grid, (hipStream_t)stream, *d, c, cpb);
   OSAR_LAUNCHED("facet_bragg_kernel");
   return OSAR_OK;
 }
