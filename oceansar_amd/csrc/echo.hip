@@ -243,13 +243,12 @@ struct FacetIn {
 
 // idx4: byte offset of the facet in a field plane; f0, f1: the lane's byte
 // offsets of its snapshots m and m + 1 (f1 = f0 at a snapshot pulse, where the
-// second value is not used) from the group's first snapshot.
-template <int NPOL, bool SPEC, bool XTI>
-__device__ __forceinline__ void load_facet(const osar_echo_desc& d, const FacetBases<NPOL>& B,
-                                           int x, uint32_t idx4, uint32_t f0, uint32_t f1, int p,
-                                           bool bragg, const FacetIdx& fi,
-                                           FacetIn<NPOL, SPEC>& f) {
-  (void)x;
+// second value is not used) from the group's first snapshot. The displacement
+// fields are loaded as soon as the column before has used them; the Bragg and
+// specular inputs once the column before has formed its scene value.
+template <int NPOL, bool SPEC>
+__device__ __forceinline__ void load_fields(const FacetBases<NPOL>& B, uint32_t idx4,
+                                            uint32_t f0, uint32_t f1, FacetIn<NPOL, SPEC>& f) {
   const uint32_t o0 = idx4 + f0, o1 = idx4 + f1;
   f.dx0 = load_at(B.dx, o0);
   f.dy0 = load_at(B.dy, o0);
@@ -257,6 +256,15 @@ __device__ __forceinline__ void load_facet(const osar_echo_desc& d, const FacetB
   f.dx1 = load_at(B.dx, o1);
   f.dy1 = load_at(B.dy, o1);
   f.dz1 = load_at(B.dz, o1);
+}
+
+// bragg: the call's Bragg flag (uniform). Lanes whose pulse has no RCS refresh
+// in the call's slots read slot 0 and do not use the values.
+template <int NPOL, bool SPEC>
+__device__ __forceinline__ void load_scat(const osar_echo_desc& d, const FacetBases<NPOL>& B,
+                                          uint32_t idx4, uint32_t f0, uint32_t f1, int p,
+                                          bool bragg, const FacetIdx& fi,
+                                          FacetIn<NPOL, SPEC>& f) {
   if (bragg) {
 #pragma unroll
     for (int pol = 0; pol < NPOL; ++pol) {
@@ -267,6 +275,7 @@ __device__ __forceinline__ void load_facet(const osar_echo_desc& d, const FacetB
     f.sp = d.scat_p[(uint32_t)(fi.p + p)];
   }
   if (SPEC) {
+    const uint32_t o0 = idx4 + f0, o1 = idx4 + f1;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       f.s0[k] = load_at(B.s[k], o0);
@@ -488,25 +497,25 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
       for (int j = 0; j < kTaps; ++j) win[pol][j] = make_float2(0.f, 0.f);
     int wbase = INT_MIN;
 
-    // software pipeline: inputs of column x + 1 and offsets of column x + 2
-    // are in flight while column x is processed
-    FacetIn<NPOL, SPEC> cur, nxt;
-    FacetIdx ia = {0, 0}, ib = {0, 0};
+    // software pipeline: each input of column x + 1 is loaded into the
+    // register that held column x's value as soon as column x has used it
+    // (the last column reloads itself), the scatterer offsets two columns
+    // ahead, so the loop carries no register copies
+    FacetIn<NPOL, SPEC> cur;
+    FacetIdx ia = {0, 0};
     const uint32_t col4 = (uint32_t)d.ny * 4u;   // bytes between columns of a field plane
     const uint32_t idx4_lo = ((uint32_t)x_lo * d.ny + yrow) * 4u;
-    load_index(d, idx4_lo, bragg_ok, ia);
-    load_facet<NPOL, SPEC, XTI>(d, B, x_lo, idx4_lo, f0, f1, p, bragg_ok, ia, cur);
-    if (x_lo + 1 < x_hi) load_index(d, idx4_lo + col4, bragg_ok, ia);
+    load_index(d, idx4_lo, bragg, ia);
+    load_fields<NPOL, SPEC>(B, idx4_lo, f0, f1, cur);
+    load_scat<NPOL, SPEC>(d, B, idx4_lo, f0, f1, p, bragg, ia, cur);
+    load_index(d, idx4_lo + (uint32_t)(min(x_lo + 1, x_hi - 1) - x_lo) * col4, bragg, ia);
     for (int x = x_lo; x < x_hi; ++x) {
-      const uint32_t idx4 = idx4_lo + (uint32_t)(x - x_lo) * col4;
-      if (x + 1 < x_hi) {
-        ib = ia;
-        if (x + 2 < x_hi) load_index(d, idx4 + 2 * col4, bragg_ok, ia);
-        load_facet<NPOL, SPEC, XTI>(d, B, x + 1, idx4 + col4, f0, f1, p, bragg_ok, ib, nxt);
-      }
+      const uint32_t idx4n = idx4_lo + (uint32_t)(min(x + 1, x_hi - 1) - x_lo) * col4;
+      const uint32_t idx4nn = idx4_lo + (uint32_t)(min(x + 2, x_hi - 1) - x_lo) * col4;
       const double Dx = lerp2(cur.dx0, cur.dx1, rk, wl, wh);
       const double Dy = lerp2(cur.dy0, cur.dy1, rk, wl, wh);
       const double Dz = lerp2(cur.dz0, cur.dz1, rk, wl, wh);
+      load_fields<NPOL, SPEC>(B, idx4n, f0, f1, cur);
       const double c_sr = uniform_load(d.sr_col, x), c_ci = uniform_load(d.cos_inc, x),
                    c_si = uniform_load(d.sin_inc, x);
       const Geo g = facet_range(c_sr, c_ci, c_si, mig, sin_az, Dx, Dy,
@@ -553,6 +562,8 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
           }
         }
       }
+      load_scat<NPOL, SPEC>(d, B, idx4n, f0, f1, p, bragg, ia, cur);
+      load_index(d, idx4nn, bragg, ia);
       if (finite) {
         if (!XTI) {
           // slide the window so that it starts at bin dbin - 4
@@ -621,7 +632,6 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
           }
         }
       }
-      cur = nxt;
     }
     if (!XTI && wbase != INT_MIN) {
 #pragma unroll
