@@ -71,6 +71,7 @@ struct SynthArgs {
   const double2* twx;   // [128]: exp(2 pi i k / nx) for k < 64, then for k = 64 m
   const double2* twy;
   double2* work;
+  double* omega;        // [ny][nx] omega(|k|), filled by omega_kernel
   float* out[8];
   QPlan q[kMaxQ];
 };
@@ -387,9 +388,22 @@ constexpr int kColRadixLog = 3;                  // radix-8 passes (one butterfl
 constexpr int kWsBlock = 8;                      // workspace run: 8 columns = 128 bytes
 constexpr int kRowPairs = kMaxN / kRowThreads;   // column pairs a row thread owns
 
-// a(k) exp(-i omega(|k|) t) for the complex64 coefficient c and wave number k
-__device__ __forceinline__ double2 phased(osar_c32 c, double k, double t, double depth) {
-  const double om = depth > 0. ? sqrt(kG * k * tanh(k * depth)) : sqrt(kG * k);
+// Dispersion relation omega(|k|) over the wave-number grid, once per call: the
+// row pass reads it instead of evaluating sqrt and tanh per snapshot (the same
+// expression, so the same bits).
+__global__ void omega_kernel(SynthArgs a) {
+  const int64_t n = (int64_t)a.nx * a.ny;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e >> a.logx), i = (int)(e & (a.nx - 1));
+    const double x = a.kx[i], y = a.ky[j];
+    const double k = sqrt(x * x + y * y);
+    a.omega[e] = a.depth > 0. ? sqrt(kG * k * tanh(k * a.depth)) : sqrt(kG * k);
+  }
+}
+
+// c exp(-i omega t) for the complex64 coefficient c
+__device__ __forceinline__ double2 phased(osar_c32 c, double om, double t) {
   double sn, cs;
   sincos(-(om * t), &sn, &cs);
   return cmul2(make_double2((double)c.re, (double)c.im), make_double2(cs, sn));
@@ -398,23 +412,20 @@ __device__ __forceinline__ double2 phased(osar_c32 c, double k, double t, double
 // The phased coefficients A0 = a(k) at (row j, column i) and A1 = a(k') at the
 // mirror (jm, im); k' has the same |k| on a 2 pi fftfreq grid (kx(-i) = -kx(i)
 // except at the Nyquist column, where it equals kx(i)), so omega, the phase and
-// its sincos are shared then. Returns |k| at (j, i).
-__device__ __forceinline__ double phased_pair(const SynthArgs& a, int j, int i, int jm, int im,
-                                              double t, double2& A0, double2& A1) {
-  const double x0 = a.kx[i], x1 = a.kx[im], y0 = a.ky[j], y1 = a.ky[jm];
-  const double k0 = sqrt(x0 * x0 + y0 * y0);
-  const double k1 = sqrt(x1 * x1 + y1 * y1);
+// its sincos are shared then (equal omega gives equal bits either way).
+__device__ __forceinline__ void phased_pair(const SynthArgs& a, int j, int i, int jm, int im,
+                                            double t, double2& A0, double2& A1) {
+  const double om0 = a.omega[(size_t)j * a.nx + i];
+  const double om1 = a.omega[(size_t)jm * a.nx + im];
   const osar_c32 c0 = a.coefs[(size_t)j * a.nx + i];
   const osar_c32 c1 = a.coefs[(size_t)jm * a.nx + im];
-  const double om0 = a.depth > 0. ? sqrt(kG * k0 * tanh(k0 * a.depth)) : sqrt(kG * k0);
   double sn, cs;
   sincos(-(om0 * t), &sn, &cs);
   A0 = cmul2(make_double2((double)c0.re, (double)c0.im), make_double2(cs, sn));
-  if (k1 == k0)
+  if (om1 == om0)
     A1 = cmul2(make_double2((double)c1.re, (double)c1.im), make_double2(cs, sn));
   else
-    A1 = phased(c1, k1, t, a.depth);
-  return k0;
+    A1 = phased(c1, om1, t);
 }
 
 // Pass 1: one workgroup per row pair (j, -j) and snapshot. Thread t owns the
@@ -540,7 +551,8 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
         A1 = S[pad(nx + im)];
         k0 = sqrt(kx * kx + ky * ky);
       } else {
-        k0 = phased_pair(a, j, i, jm, im, t, A0, A1);
+        phased_pair(a, j, i, jm, im, t, A0, A1);
+        k0 = sqrt(kx * kx + ky * ky);
       }
       const double ik = k0 > a.thr ? 1. / k0 : 0.;
       const double km = sqrt(kxm * kxm + kym * kym);
@@ -732,7 +744,9 @@ extern "C" int64_t osar_surface_workspace_bytes(const osar_surface_desc* d) {
   QPlan q[kMaxQ];
   int nq = 0;
   if (make_plan(d, q, &nq) != OSAR_OK) return -1;
-  return (int64_t)nq * d->n_times * (int64_t)d->nx * d->ny * (int64_t)sizeof(double2);
+  // packed spectra [nq][n_times][ny][nx], then the omega table [ny][nx]
+  return (int64_t)nq * d->n_times * (int64_t)d->nx * d->ny * (int64_t)sizeof(double2) +
+         (int64_t)d->nx * d->ny * (int64_t)sizeof(double);
 }
 
 extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream) {
@@ -777,6 +791,7 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   a.coefs = d->coefs;
   a.times = d->times;
   a.work = (double2*)ctx->work;
+  a.omega = (double*)(a.work + (size_t)a.nq * d->n_times * d->nx * d->ny);
   int rc = twiddles(ctx, d->nx, &a.twx);
   if (rc) return rc;
   rc = twiddles(ctx, d->ny, &a.twy);
@@ -807,6 +822,12 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
     if (rc2) return rc2;
   }
   a.n_times = d->n_times;
+  {
+    const int64_t n = (int64_t)d->nx * d->ny;
+    const int blocks = (int)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
+    hipLaunchKernelGGL(omega_kernel, dim3(blocks), dim3(256), 0, s, a);
+    OSAR_LAUNCHED("omega_kernel");
+  }
   const dim3 grid_rows((d->ny / 2 + 1) * d->n_times), grid_cols((d->nx / cb) * a.nq * d->n_times);
   if (short_x && mirror)
     hipLaunchKernelGGL((synth_rows_kernel<true, true>), grid_rows, dim3(kRowThreads), lds_rows, s,
