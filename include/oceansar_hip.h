@@ -90,7 +90,8 @@ typedef struct osar_echo_desc {
   int32_t cols_per_block;   /* columns per workgroup (0 = all); below nx, sr_col must
                                be non-decreasing (the window starts at the chunk's
                                first column) */
-  int32_t reserved;
+  int32_t scat_len;         /* entries of scat_m, scat_p (N + n_pulses_total; the
+                               larger if they differ), at most 2^29 (OSAR_ECHO_BRAGG) */
   /* FP64 scalars */
   double t_step, t_span, v_ground, sr0, gr0, k0;
   double min_sr, binsize;   /* bin position x = (sr - min_sr) / binsize */

@@ -29,7 +29,7 @@ class EchoDesc(ctypes.Structure):
     _fields_ = [(n, c_int32) for n in (
         'nx', 'ny', 'n_pulses_total', 'pulse0', 'npulses', 'nbins', 'n_ch', 'n_pol', 'K',
         'snap0', 'n_snap_slots', 'rcs0', 'n_rcs_slots', 'flags', 'pulses_per_block',
-        'rows_per_block', 'window_bins', 'pol', 'cols_per_block', 'reserved')] + \
+        'rows_per_block', 'window_bins', 'pol', 'cols_per_block', 'scat_len')] + \
         [(n, c_double) for n in (
             't_step', 't_span', 'v_ground', 'sr0', 'gr0', 'k0', 'min_sr', 'binsize',
             'ltx_over_l', 'lrx_over_l', 'sr_col_min', 'sr_col_max', 'resid_max')] + \

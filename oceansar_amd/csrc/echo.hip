@@ -271,8 +271,9 @@ __device__ __forceinline__ void load_scat(const osar_echo_desc& d, const FacetBa
       f.amp[pol][0] = load_at(B.amp[pol][0], idx4);
       f.amp[pol][1] = load_at(B.amp[pol][1], idx4);
     }
-    f.sm = d.scat_m[(uint32_t)(fi.m + p)];
-    f.sp = d.scat_p[(uint32_t)(fi.p + p)];
+    // byte offsets below 2^32: the host bounds the series to 2^29 entries
+    f.sm = load_at(d.scat_m, (uint32_t)(fi.m + p) << 3);
+    f.sp = load_at(d.scat_p, (uint32_t)(fi.p + p) << 3);
   }
   if (SPEC) {
     const uint32_t o0 = idx4 + f0, o1 = idx4 + f1;
@@ -790,6 +791,10 @@ extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void
     OSAR_PTR(d->ind0_m);
     OSAR_PTR(d->ind0_p);
     OSAR_PTR(d->cap_rate);
+    // scatterer values are gathered at 32-bit byte offsets
+    OSAR_REQUIRE(d->scat_len > d->n_pulses_total && d->scat_len <= (1 << 29), OSAR_EINVAL,
+                 "scat_len %d: the scatterer series need n_pulses_total < entries <= 2^29",
+                 d->scat_len);
   }
   if (d->flags & OSAR_ECHO_XTI) OSAR_PTR(d->dinc_col);
   if (d->flags & OSAR_ECHO_SPECULAR) {

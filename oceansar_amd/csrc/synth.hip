@@ -412,11 +412,13 @@ __device__ __forceinline__ double2 phased(osar_c32 c, double om, double t) {
 // The phased coefficients A0 = a(k) at (row j, column i) and A1 = a(k') at the
 // mirror (jm, im); k' has the same |k| on a 2 pi fftfreq grid (kx(-i) = -kx(i)
 // except at the Nyquist column, where it equals kx(i)), so omega, the phase and
-// its sincos are shared then (equal omega gives equal bits either way).
+// its sincos are shared then (equal omega gives equal bits either way). MIRROR:
+// the tables are such a grid, so the mirror's omega is not read.
+template <bool MIRROR>
 __device__ __forceinline__ void phased_pair(const SynthArgs& a, int j, int i, int jm, int im,
                                             double t, double2& A0, double2& A1) {
   const double om0 = a.omega[(size_t)j * a.nx + i];
-  const double om1 = a.omega[(size_t)jm * a.nx + im];
+  const double om1 = MIRROR ? om0 : a.omega[(size_t)jm * a.nx + im];
   const osar_c32 c0 = a.coefs[(size_t)j * a.nx + i];
   const osar_c32 c1 = a.coefs[(size_t)jm * a.nx + im];
   double sn, cs;
@@ -476,7 +478,7 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
     const int i = min(tid + u * kRowThreads, nx - 1);
     const int im = (nx - i) & (nx - 1);
     double2 A0, A1;
-    phased_pair(a, j, i, jm, im, t, A0, A1);
+    phased_pair<MIRROR>(a, j, i, jm, im, t, A0, A1);
     H[u] = make_double2(0.5 * (A0.x + A1.x), 0.5 * (A0.y - A1.y));
     if (i == nx / 2 && tid + u * kRowThreads < nx) {   // only this thread reads them back
       nyq[0] = A0;
@@ -551,7 +553,7 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
         A1 = S[pad(nx + im)];
         k0 = sqrt(kx * kx + ky * ky);
       } else {
-        phased_pair(a, j, i, jm, im, t, A0, A1);
+        phased_pair<MIRROR>(a, j, i, jm, im, t, A0, A1);
         k0 = sqrt(kx * kx + ky * ky);
       }
       const double ik = k0 > a.thr ? 1. / k0 : 0.;

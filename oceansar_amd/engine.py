@@ -365,6 +365,7 @@ class DeviceScene:
             d.amp = _lib.ptr(amp)
             d.rcs_slot = _lib.ptr(self.rcs_slot)
             d.scat_m, d.scat_p = _lib.ptr(self.scat_m), _lib.ptr(self.scat_p)
+            d.scat_len = max(self.scat_m.numel(), self.scat_p.numel())
             d.ind0_m, d.ind0_p = _lib.ptr(self.ind0_m), _lib.ptr(self.ind0_p)
         return d
 
