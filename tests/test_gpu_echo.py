@@ -143,6 +143,33 @@ def test_accumulator_launch_shapes_agree(need_gpu, cfg1_scene):
         assert rel_rms(o, outs[0]) < 2e-6
 
 
+def test_accumulator_rejects_scatterer_series_beyond_32bit_offsets(need_gpu, cfg1_scene):
+    """The scatterer gathers use 32-bit byte offsets: a series of more than 2^29
+    entries, or one too short for the pulses, is refused before any launch."""
+    import ctypes
+    import torch
+    from oceansar_amd import _lib
+    from oceansar_amd.engine import DeviceScene
+    sc = cfg1_scene
+    ds = DeviceScene(sc)
+    p0, n = 0, 8
+    m1 = (p0 + n - 1) // sc.K + 1
+    f = torch.zeros((m1 + 1, sc.Nx, sc.Ny), dtype=torch.float32, device='cuda')
+    amp = torch.zeros((1, 1, 2, sc.Nx, sc.Ny), dtype=torch.float32, device='cuda')
+    raw = ds.new_raw()
+    d = ds.echo_desc(f, f, f, 0, amp, int(sc.rcs_slot[p0]), p0, n, raw)
+    assert d.scat_len == ds.scat_m.numel() > ds.n_pulses
+    lib = ds.ctx.lib
+    # the descriptor is otherwise valid (zero amplitudes: nothing is added)
+    assert lib.osar_echo_accumulate(ds.ctx.h, ctypes.byref(d), _lib.stream_ptr()) == 0
+    for bad in ((1 << 29) + 1, ds.n_pulses):
+        d.scat_len = bad
+        assert lib.osar_echo_accumulate(ds.ctx.h, ctypes.byref(d), _lib.stream_ptr()) != 0
+        assert 'scat_len' in lib.osar_last_error().decode()
+    torch.cuda.synchronize()
+    assert float(raw.abs().max()) == 0.0
+
+
 def test_deposit_matches_oracle(need_gpu):
     import torch
     import oracle.echo as oecho
