@@ -36,6 +36,8 @@ struct BraggConst {
   double sq_re, sq_im;           // sqrt(epsilon_sw), principal branch
   double dxdy;                   // float32(dx) * float32(dy) as the reference multiplies
   double inv_cK_p;               // 1 / cK_p
+  double inv_cK_p25, cK_m25;     // cK_p^-2.5, cK_m^2.5
+  double inv_2k0;                // 1 / (2 k0)
 };
 
 BraggConst bragg_const(const osar_bragg_desc& d) {
@@ -59,11 +61,11 @@ BraggConst bragg_const(const osar_bragg_desc& d) {
   c.sq_im = 18. / (2. * c.sq_re);
   c.dxdy = (double)(float)d.dx * (double)(float)d.dy;
   c.inv_cK_p = 1. / c.cK_p;
+  c.inv_cK_p25 = std::pow(c.cK_p, -2.5);
+  c.cK_m25 = std::pow(c.cK_m, 2.5);
+  c.inv_2k0 = 1. / (2. * d.k0);
   return c;
 }
-
-// x^2.5 as x^2 sqrt(x) (the same value as pow(x, 2.5) to rounding)
-__device__ __forceinline__ double pow25(double x) { return x * x * sqrt(x); }
 
 // (cos, sin) of atan(t): (1, t) / sqrt(1 + t^2). Beyond |t| = 1e150 (where
 // t^2 would overflow) cos < 1e-150 is returned as 0 and sin as sign t, the
@@ -75,11 +77,15 @@ __device__ __forceinline__ void cos_sin_atan(double t, double* c, double* s) {
   *s = huge ? copysign(1., t) : t * r;
 }
 
-// a / (d d) for complex a, d as a conj(d d) / |d d|^2: one real division
-__device__ __forceinline__ cd cdiv_sq(cd a, cd den) {
-  const cd q = cmulc(den, den);
-  const double inv = 1. / (q.re * q.re + q.im * q.im);
-  return cscale(cmulc(a, {q.re, -q.im}), inv);
+// a1 / (d1 d1) and a2 / (d2 d2) for complex a, d as a conj(q) / |q|^2 with
+// q = d d; the two reciprocals 1 / |q1|^2, 1 / |q2|^2 come from one real
+// division 1 / (|q1|^2 |q2|^2) (|d| is between 8 and 100 here: no overflow)
+__device__ __forceinline__ void cdiv_sq2(cd a1, cd den1, cd a2, cd den2, cd* r1, cd* r2) {
+  const cd q1 = cmulc(den1, den1), q2 = cmulc(den2, den2);
+  const double n1 = q1.re * q1.re + q1.im * q1.im, n2 = q2.re * q2.re + q2.im * q2.im;
+  const double inv = 1. / (n1 * n2);
+  *r1 = cscale(cmulc(a1, {q1.re, -q1.im}), n2 * inv);
+  *r2 = cscale(cmulc(a2, {q2.re, -q2.im}), n1 * inv);
 }
 
 // Grid point i of a table built as numpy.arange(x0, ., dk): x0 + i * dk with
@@ -185,8 +191,9 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
     const double sin_ti_2 = 1 - cos_ti * cos_ti;
     const cd eps = {73., 18.};
     const cd sq = {c.sq_re, c.sq_im};
-    const cd b_hh = cdiv_sq(eps, cadd({cos_ti, 0.}, sq));
-    const cd b_vv = cdiv_sq(cscale(cmulc(eps, eps), 1. + sin_ti_2), cadd(cscale(eps, cos_ti), sq));
+    cd b_hh, b_vv;
+    cdiv_sq2(eps, cadd({cos_ti, 0.}, sq), cscale(cmulc(eps, eps), 1. + sin_ti_2),
+             cadd(cscale(eps, cos_ti), sq), &b_hh, &b_vv);
     const double wgt = cos_ip * (sec_inc[col - col_lo] * sec_sp);   // cos_ip / (cos_inc cos_sp)
     const double inv_ti = 1. / sin_ti_2;
     const double a = sin_s_t_2 * inv_ti;
@@ -195,22 +202,36 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
     const double F0 = 8 * M_PI * (kc * kc) * (kc * kc);
     const double T_vv = wgt * (F0 * cabs2(cadd(cscale(b_vv, b), cscale(b_hh, a))));
     const double T_hh = wgt * (F0 * cabs2(cadd(cscale(b_hh, b), cscale(b_vv, a))));
-    double k_b = 2 * d.k0 * sqrt(sin_ip_2 + cos_ip_2 * sin_s_t_2);
-    if (k_b > 2 * d.k0) k_b = 0.;
-    // phi_b = az + atan(q); th0 = phi_b - wind_dir; cos(th0) from angle sums
-    double cos_q, sin_q;
-    cos_sin_atan((cos_ip * -sin_st) / sin_ip, &cos_q, &sin_q);
+    // k_b = 2 k0 sqrt(zb) and 1 / k_b from one reciprocal square root
+    const double zb = sin_ip_2 + cos_ip_2 * sin_s_t_2;
+    const double rz = rsqrt(zb);   // +inf at zb = 0
+    double k_b = zb > 0. ? 2 * d.k0 * (zb * rz) : 0.;
+    double inv_kb = rz * c.inv_2k0;
+    if (k_b > 2 * d.k0) {
+      k_b = 0.;
+      inv_kb = INFINITY;
+    }
+    // phi_b = az + atan(q), q = -cos_ip sin_st / sin_ip; th0 = phi_b - wind_dir;
+    // cos(th0) from angle sums. 1 + q^2 = zb / sin_ip^2, so cos(atan q) =
+    // |sin_ip| / sqrt(zb) and sin(atan q) = -cos_ip sin_st sign(sin_ip) / sqrt(zb)
+    const double cos_q = fabs(sin_ip) * rz;
+    const double sin_q = -cos_ip * sin_st * copysign(rz, sin_ip);
     const double cth0 = cos_A * cos_q - sin_A * sin_q;
     const bool bad = 4 * sin_ip_2 < d.d * d.d;
     const double S1 = interp_fill1<UNIFORM>(k_b, d.spec_k, d.spec_S, d.spec_slope, d.n_spec,
                                             d.spec_k0, d.spec_dk, d.spec_inv_dk);
-    const double inv_kb = 1. / k_b;
     double delta = 0.;
     if (SPREAD == 0) {
-      const double cK = sqrt(kG * inv_kb + kSurfT / kRho * k_b);
+      const double y = kG * inv_kb + kSurfT / kRho * k_b;
+      const double cK = sqrt(y);
+      // (cK / cK_p)^2.5 = y sqrt(cK) cK_p^-2.5 and (cK_m / cK)^2.5 = cK_m^2.5
+      // cK^-2.5, from r = 1 / sqrt(cK); k_b = 0 gives x = +inf as before
+      const double r = rsqrt(cK);
+      const double r2 = r * r;
+      double x = c.a_0 + 4. * (y * (cK * r) * c.inv_cK_p25) + c.a_m * (c.cK_m25 * (r2 * r2 * r));
+      if (!(y < INFINITY)) x = INFINITY;
       // tanh(x) = sign(x) (1 - e) / (1 + e), e = exp(-2 |x|): one exponential,
       // no branches, within 3 ulp of the library tanh (x >= ln(2) / 4 here)
-      const double x = c.a_0 + 4. * pow25(cK * c.inv_cK_p) + c.a_m * pow25(c.cK_m / cK);
       const double e = exp(-2. * fabs(x));
       delta = copysign((1. - e) / (1. + e), x);
     }
