@@ -77,17 +77,6 @@ __device__ __forceinline__ void cos_sin_atan(double t, double* c, double* s) {
   *s = huge ? copysign(1., t) : t * r;
 }
 
-// a1 / (d1 d1) and a2 / (d2 d2) for complex a, d as a conj(q) / |q|^2 with
-// q = d d; the two reciprocals 1 / |q1|^2, 1 / |q2|^2 come from one real
-// division 1 / (|q1|^2 |q2|^2) (|d| is between 8 and 100 here: no overflow)
-__device__ __forceinline__ void cdiv_sq2(cd a1, cd den1, cd a2, cd den2, cd* r1, cd* r2) {
-  const cd q1 = cmulc(den1, den1), q2 = cmulc(den2, den2);
-  const double n1 = q1.re * q1.re + q1.im * q1.im, n2 = q2.re * q2.re + q2.im * q2.im;
-  const double inv = 1. / (n1 * n2);
-  *r1 = cscale(cmulc(a1, {q1.re, -q1.im}), n2 * inv);
-  *r2 = cscale(cmulc(a2, {q2.re, -q2.im}), n1 * inv);
-}
-
 // Grid point i of a table built as numpy.arange(x0, ., dk): x0 + i * dk with
 // the product rounded before the sum, as numpy fills it.
 __device__ __forceinline__ double arange_point(double x0, double dk, int i) {
@@ -191,17 +180,16 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
     const double sin_ti_2 = 1 - cos_ti * cos_ti;
     const cd eps = {73., 18.};
     const cd sq = {c.sq_re, c.sq_im};
-    cd b_hh, b_vv;
-    cdiv_sq2(eps, cadd({cos_ti, 0.}, sq), cscale(cmulc(eps, eps), 1. + sin_ti_2),
-             cadd(cscale(eps, cos_ti), sq), &b_hh, &b_vv);
+    // b_hh = eps / d1^2 and b_vv = eps^2 (1 + sin_ti^2) / d2^2 as u conj(q) / |q|^2
+    // (q = d^2); the reciprocals are taken below, with the facet's others
+    const cd d1 = cadd({cos_ti, 0.}, sq), d2 = cadd(cscale(eps, cos_ti), sq);
+    const cd q1 = cmulc(d1, d1), q2 = cmulc(d2, d2);
+    const double n1 = cabs2(q1), n2 = cabs2(q2);
+    const cd u_hh = cmulc(eps, {q1.re, -q1.im});
+    const cd u_vv = cmulc(cscale(cmulc(eps, eps), 1. + sin_ti_2), {q2.re, -q2.im});
     const double wgt = cos_ip * (sec_inc[col - col_lo] * sec_sp);   // cos_ip / (cos_inc cos_sp)
-    const double inv_ti = 1. / sin_ti_2;
-    const double a = sin_s_t_2 * inv_ti;
-    const double b = sin_ip_2 * cos_s_t_2 * inv_ti;
     const double kc = d.k0 * cos_ti;
     const double F0 = 8 * M_PI * (kc * kc) * (kc * kc);
-    const double T_vv = wgt * (F0 * cabs2(cadd(cscale(b_vv, b), cscale(b_hh, a))));
-    const double T_hh = wgt * (F0 * cabs2(cadd(cscale(b_hh, b), cscale(b_vv, a))));
     // k_b = 2 k0 sqrt(zb) and 1 / k_b from one reciprocal square root
     const double zb = sin_ip_2 + cos_ip_2 * sin_s_t_2;
     const double rz = rsqrt(zb);   // +inf at zb = 0
@@ -220,7 +208,7 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
     const bool bad = 4 * sin_ip_2 < d.d * d.d;
     const double S1 = interp_fill1<UNIFORM>(k_b, d.spec_k, d.spec_S, d.spec_slope, d.n_spec,
                                             d.spec_k0, d.spec_dk, d.spec_inv_dk);
-    double delta = 0.;
+    double x = 0., e = 0.;
     if (SPREAD == 0) {
       const double y = kG * inv_kb + kSurfT / kRho * k_b;
       const double cK = sqrt(y);
@@ -228,13 +216,27 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
       // cK^-2.5, from r = 1 / sqrt(cK); k_b = 0 gives x = +inf as before
       const double r = rsqrt(cK);
       const double r2 = r * r;
-      double x = c.a_0 + 4. * (y * (cK * r) * c.inv_cK_p25) + c.a_m * (c.cK_m25 * (r2 * r2 * r));
+      x = c.a_0 + 4. * (y * (cK * r) * c.inv_cK_p25) + c.a_m * (c.cK_m25 * (r2 * r2 * r));
       if (!(y < INFINITY)) x = INFINITY;
       // tanh(x) = sign(x) (1 - e) / (1 + e), e = exp(-2 |x|): one exponential,
       // no branches, within 3 ulp of the library tanh (x >= ln(2) / 4 here)
-      const double e = exp(-2. * fabs(x));
-      delta = copysign((1. - e) / (1. + e), x);
+      e = exp(-2. * fabs(x));
     }
+    // One division gives 1 / |q1|^2, 1 / |q2|^2, 1 / sin_ti^2 and (Elfouhaily)
+    // 1 / (1 + e): |q| is between 8^4 and 100^4, 1 + e in [1, 2], so the
+    // product neither overflows nor underflows; each reciprocal is within a few
+    // ulp of its own division
+    const double ope = SPREAD == 0 ? 1. + e : 1.;
+    const double n12 = n1 * n2;
+    const double inv = 1. / (n12 * (sin_ti_2 * ope));
+    const double inv_ti = (n12 * ope) * inv;
+    const double inv_n12 = (sin_ti_2 * ope) * inv;
+    const cd b_hh = cscale(u_hh, n2 * inv_n12), b_vv = cscale(u_vv, n1 * inv_n12);
+    const double delta = SPREAD == 0 ? copysign((1. - e) * ((n12 * sin_ti_2) * inv), x) : 0.;
+    const double a = sin_s_t_2 * inv_ti;
+    const double b = sin_ip_2 * cos_s_t_2 * inv_ti;
+    const double T_vv = wgt * (F0 * cabs2(cadd(cscale(b_vv, b), cscale(b_hh, a))));
+    const double T_hh = wgt * (F0 * cabs2(cadd(cscale(b_hh, b), cscale(b_vv, a))));
     const double base = S1 * inv_kb;
     for (int comp = 0; comp < 2; ++comp) {
       const double cth = comp == 0 ? cth0 : -cth0;
