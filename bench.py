@@ -199,10 +199,11 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--workload', default='large', choices=configs_names())
     ap.add_argument('--pulses', type=int, default=None, help='shorter scene (testing only)')
-    ap.add_argument('--chunk', type=int, default=128,
-                    help='snapshot intervals per launch group (128: about 41 GB of fields, '
-                         'workspace and amplitudes on the device for the 2048^2 workload; '
-                         '32 / 64 / 256 measured 3 / 1 / -1 %% slower / slower / faster)')
+    ap.add_argument('--chunk', type=int, default=512,
+                    help='snapshot intervals per launch group (512: the 8192-pulse run of '
+                         'the 2048^2 workload in one group, about 164 GB of fields, synthesis '
+                         'workspace and amplitudes on the device; 128 / 256 measured 2.1 / '
+                         '0.9 %% slower, with 41 / 82 GB)')
     ap.add_argument('--cpu-sample', type=int, default=96)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--overlap', action='store_true',
