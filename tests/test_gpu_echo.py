@@ -4,7 +4,7 @@ Inputs are the oracle's FP64-synthesised surface snapshots (stored float32 as
 OceanSurface stores them), so these tests isolate the facet and echo kernels.
 Tolerances: Bragg amplitudes are float32 roundings of an FP64 evaluation
 (rel 2e-6); the raw cube must be within 1e-4 relative RMS of the FP64 oracle
-and is expected near 1e-6.
+(the parity bound of the path), and is expected near 1e-6.
 """
 import numpy as np
 import pytest

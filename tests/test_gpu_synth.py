@@ -6,7 +6,7 @@ the same FP64 value unless it lies within ~1e-15 relative of a float32
 rounding boundary, so we require >= 99.9 % bitwise-equal values and the rest
 within one float32 ulp-scale of the field RMS.
 End to end: raw within 1e-4 relative RMS and 1 mrad phase RMS of the FP64
-oracle, on windows that cross synthesis chunks and RCS
+oracle (the parity bound of the path), on windows that cross synthesis chunks and RCS
 refreshes.
 """
 import numpy as np
