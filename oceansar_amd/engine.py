@@ -219,7 +219,7 @@ class DeviceScene:
         """LDS of one accumulator workgroup (echo.hip: echo_lds_bytes, line_stride)."""
         nch_c = 1 if n_ch <= 1 else 2 if n_ch <= 2 else 4 if n_ch <= 4 else 8
         LS = ((2 * W + 31) // 32) * 32 + 4            # FP64 lines
-        return 8 * npol * nch_c * P * LS + 4 * 164 + 4 * P
+        return 8 * npol * nch_c * P * LS + 4 * 168 + 4 * P   # lines, tap table, window starts
 
     def default_shape(self, lds_budget=160 * 1024):
         """The plain kernels are built for two workgroups per CU (4 waves per SIMD):
