@@ -29,7 +29,13 @@ Contents and how each is pinned (see tests/test_oracle_*.py):
   sar.simulate        a1/a6/a11/a12  per-pulse loop with integer cadence;
                           built from the pinned parts above; the C
                           restatement (c/oracle_echo.c) agrees with it to 1e-12.
-                          Its own geometry and phase terms (range migration,
-                          Bragg phase rate, beam, channel phasors) have no
-                          separate closed-form pin.
+                          Its own geometry and phase terms are pinned by the
+                          radar geometry (tests/test_oracle_pulse.py): the
+                          displacement terms are the line-of-sight projections
+                          of an exact 3-D range, the migration term the
+                          hyperbolic range history to second order, the Bragg
+                          phase rate the capillary-gravity frequency at k_b,
+                          the beam a product of uniform-aperture patterns
+                          (quadrature), the along-track channel phase the
+                          one-way path difference b sin(az).
 """

@@ -1,0 +1,177 @@
+"""Pinning the oracle's per-pulse geometry and phase terms
+(oceansar/radarsim/sar_raw_nompi.py:352-369, 470-495, 513-518) with closed
+forms of the radar geometry, independent of the formulas the oracle restates:
+
+- a facet's displacement terms are the line-of-sight projections of its exact
+  3-D range (finite differences of |P - S| in a flat-earth frame with the
+  column's incidence), and the migration term is the second-order expansion of
+  the hyperbolic range history sqrt(sr0^2 + az^2) - sr0;
+- the Bragg phase advances per pulse by the Bragg wave's angular frequency
+  (capillary-gravity dispersion at k_b = 2 k0 sin(inc)) times the pulse
+  interval, and the minus component pairs with the minus scatterer series;
+- the two-way beam vanishes at a uniform aperture's first null, sin(az) = lambda/L;
+- an along-track channel displaced by b sees a facet with the phase
+  -k0 b sin(az) (one-way path difference b sin(az)) relative to channel 0.
+"""
+import copy
+import types
+
+import numpy as np
+import pytest
+
+import oracle.sar as osar
+
+
+@pytest.fixture(scope='module')
+def scene(tmp_path_factory):
+    from oceansar_amd import configs
+    return configs.load_scene(configs.workload('small', num_ch=2, b_ati=[0., 10.]),
+                              tmp_path_factory.mktemp('pulse'))
+
+
+def _flat(sc):
+    """Zero displacement fields (oracle layout [Ny][Nx], float32)."""
+    return {n: np.zeros((sc.Ny, sc.Nx), np.float32) for n in ('Dx', 'Dy', 'Dz')}
+
+
+def _flat_one(sc, name, row, col, value):
+    """Zero fields but `name` = value at one facet."""
+    f = _flat(sc)
+    f[name][row, col] = value
+    return f
+
+
+def _range_at(sc, fields, p):
+    cache = osar.SnapshotCache(sc)
+    m = p // sc.K
+    cache.cache[m] = fields
+    cache.cache[m + 1] = fields
+    plain = copy.copy(sc)
+    plain.bragg_enable = False
+    plain.spec_enable = False
+    sr_surface, _, sin_az, _, _ = osar.pulse_scene(plain, cache, p, None)
+    return sr_surface, sin_az
+
+
+def test_displacement_terms_are_line_of_sight_projections(scene):
+    sc = scene
+    p = (sc.az_steps // 2 // sc.K) * sc.K              # a snapshot pulse near mid-aperture
+    col = sc.Nx // 3
+    az_now = (p * sc.t_step - sc.t_span / 2.) * sc.v_ground
+    row = int(np.argmax(np.abs(sc.y64 - az_now)))      # the largest azimuth offset
+    a = float(sc.y64[row] - az_now)
+    assert abs(a) > 50.
+    base, _ = _range_at(sc, _flat(sc), p)
+    d = 0.01
+    d32 = float(np.float32(d))
+    # exact range from a platform at ground range g, height H (the column's
+    # incidence at the reference range sr0) and along-track offset a
+    inc = float(sc.inc[col])
+    g, H = sc.sr0 * np.sin(inc), sc.sr0 * np.cos(inc)
+    r = lambda gx, ay, z: np.sqrt(gx * gx + ay * ay + (H - z) ** 2)
+    exact = {'Dx': (r(g + d32, a, 0.) - r(g, a, 0.)) / d32,
+             'Dy': (r(g, a + d32, 0.) - r(g, a, 0.)) / d32,
+             'Dz': (r(g, a, d32) - r(g, a, 0.)) / d32}
+    for name in ('Dx', 'Dy', 'Dz'):
+        sr, _ = _range_at(sc, _flat_one(sc, name, row, col, d), p)
+        got = (sr[row, col] - base[row, col]) / d32
+        # first-order projections: agree to O(a^2 / sr0^2) and O(d / sr0)
+        assert abs(got - exact[name]) < 1e-5, (name, got, exact[name])
+        # every other facet is unchanged
+        mask = np.ones_like(sr, bool)
+        mask[row, col] = False
+        assert np.array_equal(sr[mask], base[mask])
+
+
+def test_migration_is_the_hyperbolic_range_to_second_order(scene):
+    sc = scene
+    p = (sc.az_steps // 4 // sc.K) * sc.K
+    sr_surface, sin_az = _range_at(sc, _flat(sc), p)
+    az_now = (p * sc.t_step - sc.t_span / 2.) * sc.v_ground
+    az = sc.y64 - az_now
+    mig = sr_surface[:, 0] - sc.sr[0]
+    hyper = np.sqrt(sc.sr0 ** 2 + az ** 2) - sc.sr0
+    # az^4 / (8 sr0^3) is the first neglected term
+    assert np.max(np.abs(az)) > 100.
+    assert np.max(np.abs(mig - hyper) - az ** 4 / (8 * sc.sr0 ** 3)) < 1e-9
+    # sin(az) is the along-track direction cosine at the reference range
+    np.testing.assert_allclose(sin_az[:, 0], az / sc.sr0, rtol=1e-15)
+
+
+def test_bragg_phase_rate_is_the_bragg_wave_frequency(scene):
+    sc = scene
+    k_b = 2. * sc.k0 * sc.sin_inc
+    omega_b = np.sqrt(9.80665 * k_b + 0.072 / 1000. * k_b ** 3)   # capillary-gravity waves
+    np.testing.assert_allclose(sc.cap_rate, omega_b * sc.t_step, rtol=1e-13)
+
+
+def _bragg_scene(sc, p, s_m, s_p):
+    fake = copy.copy(sc)
+    fake.spec_enable = False
+    fake.bragg_enable = True
+    fake.pols = ['vv']
+    n = 7
+    fake.scat_m = types.SimpleNamespace(s=np.full(n, s_m, complex), N=n,
+                                        ind0=np.zeros((sc.Ny, sc.Nx), np.int64))
+    fake.scat_p = types.SimpleNamespace(s=np.full(n, s_p, complex), N=n,
+                                        ind0=np.zeros((sc.Ny, sc.Nx), np.int64))
+    cache = osar.SnapshotCache(fake)
+    m = p // sc.K
+    cache.cache[m] = cache.cache[m + 1] = _flat(sc)
+    rcs = {'vv': np.ones((sc.Ny, sc.Nx))}
+    sr_surface, _, _, scenes, _ = osar.pulse_scene(fake, cache, p, rcs)
+    return sr_surface, scenes['vv']
+
+
+def test_bragg_scene_pairs_minus_component_with_minus_scatterers(scene):
+    sc = scene
+    p = 3 * sc.K
+    cap = sc.cap_rate.reshape(1, -1) * (p + 1)
+    sr, both = _bragg_scene(sc, p, 1., 1.)
+    surf = -2. * sc.k0 * sr
+    np.testing.assert_allclose(both, 2. * np.cos(cap) * np.exp(1j * surf), rtol=0, atol=1e-9)
+    sr, minus = _bragg_scene(sc, p, 1., 0.)
+    np.testing.assert_allclose(minus, np.exp(1j * (surf - cap)), rtol=0, atol=1e-9)
+    # the minus component's phase falls by the Bragg frequency per pulse
+    sr2, minus2 = _bragg_scene(sc, p + 1, 1., 0.)
+    step = np.angle(minus2 * np.conj(minus)) + 2. * sc.k0 * (sr2 - sr)
+    np.testing.assert_allclose(np.angle(np.exp(1j * step)),
+                               np.angle(np.exp(-1j * sc.cap_rate.reshape(1, -1))) +
+                               np.zeros_like(step), atol=1e-9)
+
+
+def _aperture(L, lam, s, n=4000):
+    """Far-field pattern of a uniformly illuminated aperture of length L,
+    (1/L) int exp(i 2 pi x s / lam) dx over [-L/2, L/2], by the midpoint rule."""
+    x = (np.arange(n) + 0.5) / n * L - L / 2.
+    return np.mean(np.exp(2j * np.pi * np.outer(s, x) / lam), axis=1).real
+
+
+def test_two_way_beam_is_the_product_of_aperture_patterns(scene):
+    sc = scene
+    lam = 299792458.0 / sc.f0
+    s = np.array([0., lam / sc.ant_l_tx, lam / sc.ant_l_rx, -lam / sc.ant_l_tx])
+    b = osar.beam_pattern(sc, s)
+    assert b[0] == 1.
+    assert abs(b[1]) < 1e-15 and abs(b[2]) < 1e-15 and abs(b[3]) < 1e-15
+    x = np.linspace(-3. * lam / sc.ant_l_tx, 3. * lam / sc.ant_l_tx, 61)
+    ref = _aperture(sc.ant_l_tx, lam, x) * _aperture(sc.ant_l_rx, lam, x)
+    np.testing.assert_allclose(osar.beam_pattern(sc, x), ref, rtol=0, atol=1e-6)
+
+
+def test_along_track_channel_phase(scene, monkeypatch):
+    sc = scene
+    assert sc.num_ch == 2 and sc.b_ati[0] == 0. and sc.b_ati[1] != 0.
+    p = 5 * sc.K
+    row, col = sc.Ny // 5, sc.Nx // 2
+    one = np.zeros((sc.Ny, sc.Nx))
+    one[row, col] = 1.
+    monkeypatch.setattr(osar, 'bragg_rcs', lambda s, snaps, pr: {pol: one for pol in s.pols})
+    raw, _ = osar.simulate(sc, p, p + 1)
+    l0, l1 = raw[0, 0, 0], raw[0, 1, 0]
+    keep = np.abs(l0) > 1e-3 * np.abs(l0).max()
+    assert keep.sum() >= 4
+    az_now = (p * sc.t_step - sc.t_span / 2.) * sc.v_ground
+    sin_az = (sc.y64[row] - az_now) / sc.sr0
+    expect = np.exp(-1j * sc.k0 * sc.b_ati[1] * sin_az)
+    np.testing.assert_allclose(l1[keep] / l0[keep], expect, rtol=1e-9)
