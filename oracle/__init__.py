@@ -37,5 +37,8 @@ Contents and how each is pinned (see tests/test_oracle_*.py):
                           phase rate the capillary-gravity frequency at k_b,
                           the beam a product of uniform-aperture patterns
                           (quadrature), the along-track channel phase the
-                          one-way path difference b sin(az).
+                          one-way path difference b sin(az), the cross-track
+                          phase b sin(inc - inc0) and its vertical-displacement
+                          turn. The reference's horizontal term Dx sin(inc)/sr0
+                          of wave_dinc is restated as written, not pinned.
 """

@@ -175,3 +175,44 @@ def test_along_track_channel_phase(scene, monkeypatch):
     sin_az = (sc.y64[row] - az_now) / sc.sr0
     expect = np.exp(-1j * sc.k0 * sc.b_ati[1] * sin_az)
     np.testing.assert_allclose(l1[keep] / l0[keep], expect, rtol=1e-9)
+
+
+def test_cross_track_channel_phase(tmp_path_factory, monkeypatch):
+    """A receiver displaced by b across track sees a facet at incidence inc
+    with the one-way path difference b sin(inc - inc0) (far field), and a
+    vertical displacement Dz turns the look direction by Dz sin(inc) / R. The
+    reference's horizontal term Dx sin(inc) / sr0 (sar_raw_nompi.py:369) is
+    kept as the reference writes it and is not pinned here."""
+    from oceansar_amd import configs
+    sc = configs.load_scene(configs.workload('small', num_ch=2, b_xti=[0., 10.]),
+                            tmp_path_factory.mktemp('xti'))
+    assert sc.b_xti[0] == 0. and sc.b_xti[1] != 0. and not np.any(sc.b_ati)
+    p = 5 * sc.K
+    row, col = sc.Ny // 3, 0
+    one = np.zeros((sc.Ny, sc.Nx))
+    one[row, col] = 1.
+    monkeypatch.setattr(osar, 'bragg_rcs', lambda s, snaps, pr: {pol: one for pol in s.pols})
+
+    def ratio(dz):
+        cache = osar.SnapshotCache(sc)
+        f = _flat_one(sc, 'Dz', row, col, dz)
+        cache.cache[p // sc.K] = cache.cache[p // sc.K + 1] = f
+        raw, _ = osar.simulate(sc, p, p + 1, snaps=cache)
+        l0, l1 = raw[0, 0, 0], raw[0, 1, 0]
+        keep = np.abs(l0) > 1e-3 * np.abs(l0).max()
+        assert keep.sum() >= 4
+        q = l1[keep] / l0[keep]
+        assert np.allclose(q, q[0], rtol=1e-9)
+        return q[0]
+
+    b = sc.b_xti[1]
+    inc, inc0 = float(sc.inc[col]), float(sc.inc_angle)
+    assert abs(inc - inc0) > 1.5e-4
+    np.testing.assert_allclose(ratio(0.), np.exp(-1j * sc.k0 * b * np.sin(inc - inc0)),
+                               rtol=0, atol=1e-6)
+    # vertical displacement: look-angle change from the exact geometry
+    dz = float(np.float32(0.5))
+    g, H = sc.sr0 * np.sin(inc), sc.sr0 * np.cos(inc)
+    dtheta = np.arctan2(g, H - dz) - np.arctan2(g, H)
+    turn = ratio(dz) / ratio(0.)
+    np.testing.assert_allclose(turn, np.exp(-1j * sc.k0 * b * dtheta), rtol=0, atol=1e-6)
