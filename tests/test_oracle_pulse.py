@@ -216,3 +216,21 @@ def test_cross_track_channel_phase(tmp_path_factory, monkeypatch):
     dtheta = np.arctan2(g, H - dz) - np.arctan2(g, H)
     turn = ratio(dz) / ratio(0.)
     np.testing.assert_allclose(turn, np.exp(-1j * sc.k0 * b * dtheta), rtol=0, atol=1e-6)
+
+
+def test_calibration_factor_is_the_integrated_two_way_power_pattern(scene):
+    """cal_factor = 1 / sqrt(int |G_tx G_rx|^2 daz * sr_res / sin(inc0))
+    (sar_raw_nompi.py:557-563): with G = sinc(L sin(az) / lambda), the integral
+    over the simulated aperture |az| <= t_span v / 2 is sr0 lambda / L times
+    int sinc(u)^2 sinc(u L_rx / L_tx)^2 du, evaluated here by adaptive
+    quadrature instead of the trapezoid rule on the reference's grid."""
+    from scipy.integrate import quad
+    sc = scene
+    lam = 299792458.0 / sc.f0
+    scale = sc.sr0 * lam / sc.ant_l_tx                   # az per unit of u
+    U = sc.t_span / 2. * sc.v_ground / scale
+    rr = sc.ant_l_rx / sc.ant_l_tx
+    I, _ = quad(lambda u: (np.sinc(u) * np.sinc(u * rr)) ** 2, -U, U, limit=200,
+                epsabs=1e-13, epsrel=1e-12)
+    expect = 1. / np.sqrt(scale * I * sc.sr_res / np.sin(sc.inc_angle))
+    np.testing.assert_allclose(sc.cal_factor, expect, rtol=2e-4)
