@@ -242,9 +242,18 @@ def main():
     W = sc.contributions
     raw = ds.new_raw()
     overlap = args.overlap
+    # the launch group's fields, amplitudes and synthesis workspace must fit
+    # the free device memory: halve it until they take at most 90 % of it
+    from oceansar_amd.batch import footprint_bytes, workspace_bytes
+    chunk = args.chunk
+    free = torch.cuda.mem_get_info()[0]
+    nbuf = 2 if overlap else 1
+    while chunk > 8 and (nbuf * footprint_bytes(sc, chunk) + workspace_bytes(sc, chunk)
+                         > 0.9 * free):
+        chunk //= 2
     for _ in range(args.warmup):
         ds.zero_c32(raw)
-        ds.run(raw=raw, chunk_snaps=args.chunk, overlap=overlap)
+        ds.run(raw=raw, chunk_snaps=chunk, overlap=overlap)
     torch.cuda.synchronize()
 
     stage_events = []
@@ -256,7 +265,7 @@ def main():
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ds.zero_c32(raw)
-        ds.run(raw=raw, chunk_snaps=args.chunk, timer=timer, overlap=overlap)
+        ds.run(raw=raw, chunk_snaps=chunk, timer=timer, overlap=overlap)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
 
@@ -294,7 +303,8 @@ def main():
                    'facets': sc.Nx * sc.Ny, 'pulses': sc.az_steps, 'channels': sc.num_ch,
                    'range_bins': sc.rg_samp, 'contributions': W,
                    'stage_ms_per_step': {k: round(v, 3) for k, v in per_stage.items()},
-                   'echo_launch_shape': repr(ds.shape)},
+                   'echo_launch_shape': repr(ds.shape),
+                   'launch_group_intervals': chunk},
         'roofline': {'bound': 'mfma', 'kernel': 'echo_accumulate_kernel',
                      'achieved': achieved, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / FP32_PEAK_TFLOPS, 'traffic': traffic,

@@ -325,9 +325,10 @@ def footprint_bytes(sc, chunk=32):
 
 def workspace_bytes(sc, chunk=32):
     """Synthesis workspace of a context serving scenes of this size: one
-    complex128 plane per packed spectrum and snapshot of a chunk."""
+    complex128 plane per packed spectrum and snapshot of a chunk, and the
+    float64 dispersion table (osar_surface_workspace_bytes)."""
     nf = 3 + (2 if sc.bragg_enable else 0) + (5 if sc.spec_enable else 0)
     if sc.spec_enable and sc.bragg_enable:
         nf = 8
     nq = (nf + 1) // 2
-    return int(nq * (chunk + 1) * sc.Nx * sc.Ny * 16)
+    return int(nq * (chunk + 1) * sc.Nx * sc.Ny * 16 + sc.Nx * sc.Ny * 8)
