@@ -17,6 +17,10 @@ Decisions where the reference is ambiguous (recorded once here):
   * The scatterer time index is the integer pulse index (not int(t*Fs)).
   * [sar] n_pulses, when present, overrides the pulse count derived from
     t_span (sar_raw_nompi.py:267-268); t_span itself is unchanged.
+  * [ocean] frozen_ocean: the reference skips the surface time update
+    (sar_raw_nompi.py:357-358) and so reads fields it never computed
+    (np.empty, ocean.py:383-403); here a frozen surface is the surface at t = 0
+    for every pulse. The Bragg phase and the scatterer series still advance.
   * Random draws come from numpy.random.SeedSequence([sim] seed).spawn(5) in
     the order: wave coefficients, swell, scatterers minus, scatterers plus,
     noise.
@@ -251,6 +255,9 @@ class SarScene:
         return (self.az_steps - 1) // self.K + 2 if self.K > 1 else self.az_steps
 
     def snapshot_time(self, m):
+        """Time of surface snapshot m: m K t_step, or 0 for a frozen ocean."""
+        if self.frozen:
+            return 0.
         return float(m * self.K) * self.t_step
 
     def az_now(self, p):

@@ -3,7 +3,7 @@
 Mirrors oceansar/spread/__init__.py:51. On the path: Elfouhaily
 (oceansar/spread/elfouhaily.py:23-56) and cos^2 (oceansar/spread/cos2s.py:13-18).
 The same formulas run per facet inside the Bragg facet kernel
-(oceansar_amd/csrc/facet_bragg.hip); this host version is used when the
+(oceansar_amd/csrc/bragg.hip); this host version is used when the
 surface spectrum is built.
 """
 import numpy as np

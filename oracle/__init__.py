@@ -15,8 +15,17 @@ Contents and how each is pinned (see tests/test_oracle_*.py):
   surface.synth       a2  numpy complex128 ifft2 restatement; pinned by a
                           single-mode closed form (elevation, displacements,
                           slopes, curvatures, choppy forms) and Parseval.
-  bragg.rcs_romeiser97 a9 pinned by the flat-surface SPM closed form, slope
-                          symmetry, the bad-facet mask and spread half plane.
+  bragg.rcs_romeiser97 a9 facet_terms (k_b, theta, T_hh, T_vv, bad) and the
+                          Elfouhaily spreading (Delta(k) included) pinned by
+                          golden vectors written by the reference's own
+                          _rcs_numba and spread.elfouhaily source text on tilted
+                          facets (tests/golden/make_golden_bragg.py); also by
+                          the flat-surface SPM closed form, SPM at the local
+                          incidence for a pure range tilt, the tangent-plane
+                          Bragg vector of a pure azimuth tilt, slope symmetry,
+                          the bad-facet mask and spread half plane;
+                          rcs_from_terms by scipy's interp1d and the k_b = 0
+                          special case (inf / NaN before the mask).
   ka.field_spa        a10 pinned by the geometrical-optics RCS of a specular
                           point, pi R1 R2 |R(0)|^2 (caps, rotated axes, a
                           saddle, both polarisations), the facet-boundary cut
@@ -39,6 +48,8 @@ Contents and how each is pinned (see tests/test_oracle_*.py):
                           (quadrature), the along-track channel phase the
                           one-way path difference b sin(az), the cross-track
                           phase b sin(inc - inc0) and its vertical-displacement
-                          turn. The reference's horizontal term Dx sin(inc)/sr0
-                          of wave_dinc is restated as written, not pinned.
+                          turn. facet_range (sr_surface, wave_dinc, including
+                          the reference's horizontal term Dx sin(inc)/sr0 that no
+                          closed form fixes) is pinned bit for bit by golden
+                          vectors of the reference's own two statements.
 """

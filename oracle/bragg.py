@@ -101,23 +101,34 @@ def rcs_romeiser97(inc_col, az_row, diffx, diffy, k0, pol, wind_dir, d, spec_k, 
     diffx = np.asarray(diffx, dtype=np.float64)
     diffy = np.asarray(diffy, dtype=np.float64)
     k_b, th, T_hh, T_vv, bad = facet_terms(inc, az, diffx, diffy, k0, wind_dir, d)
+    return rcs_from_terms(k_b, th, T_hh, T_vv, bad, pol, spec_k, spec_S, spread_id, wind_U,
+                          wind_fetch, dx, dy)
+
+
+def rcs_from_terms(k_b, th, T_hh, T_vv, bad, pol, spec_k, spec_S, spread_id, wind_U,
+                   wind_fetch, dx, dy):
+    """RCSRomeiser97.rcs_numba after _rcs_numba (romeiser97.py:338-393): folded
+    spectrum S(k_b)/k_b G(k_b, theta), RCS = T spectrum dx dy, bad-facet mask.
+    k_b = 0 (the reference's k_b > 2 k0 rule) gives S = 1 (fill value) over 0:
+    inf where G > 0, NaN where G = 0, before the mask."""
     k_inp = np.array([k_b, k_b])
     S1 = interp_fill1(k_b.ravel(), spec_k, spec_S).reshape((1,) + k_b.shape)
     with np.errstate(divide='ignore', invalid='ignore'):
         spectrum = S1 / k_inp * SPREADS[spread_id](k_inp, th, wind_U, wind_fetch)
     out = {}
-    if pol in ('vv', 'DP'):
-        r = T_vv * spectrum * np.float32(dx) * np.float32(dy)
-        r[0][bad] = 0
-        r[1][bad] = 0
-        out['vv'] = r
-    if pol in ('hh', 'DP'):
-        r = T_hh * spectrum * np.float32(dx) * np.float32(dy)
-        if pol == 'DP':
-            # romeiser97.py:391-392: only the plus component is masked for DP
-            r[1][bad] = 0
-        else:
+    with np.errstate(invalid='ignore'):
+        if pol in ('vv', 'DP'):
+            r = T_vv * spectrum * np.float32(dx) * np.float32(dy)
             r[0][bad] = 0
             r[1][bad] = 0
-        out['hh'] = r
+            out['vv'] = r
+        if pol in ('hh', 'DP'):
+            r = T_hh * spectrum * np.float32(dx) * np.float32(dy)
+            if pol == 'DP':
+                # romeiser97.py:391-392: only the plus component is masked for DP
+                r[1][bad] = 0
+            else:
+                r[0][bad] = 0
+                r[1][bad] = 0
+            out['hh'] = r
     return out

@@ -31,7 +31,9 @@ class SnapshotCache:
         if m not in self.cache:
             if len(self.cache) > 3:
                 self.cache.pop(min(self.cache))
-            t = float(m * self.sc.K) * self.sc.t_step
+            # a frozen ocean stays at t = 0 (sar_raw_nompi.py:357-358 skips the
+            # time update; the build evaluates the surface once at t = 0)
+            t = 0. if self.sc.frozen else float(m * self.sc.K) * self.sc.t_step
             self.cache[m] = _surface.synth(self.sc.surface, t, self.compute)
         return self.cache[m]
 
@@ -78,6 +80,13 @@ def bragg_rcs(sc, snaps, p):
                                  b.wind_U, b.wind_fetch, sc.surface.dx, sc.surface.dy)
 
 
+def facet_range(sr, cos_inc, sin_inc, az, sin_az, Dx, Dy, Dz, sr0):
+    """sr_surface and wave_dinc of every facet (sar_raw_nompi.py:366-369)."""
+    sr_surface = (sr - cos_inc * Dz + az / 2 * sin_az + Dx * sin_inc + Dy * sin_az)
+    wave_dinc = (Dz * sin_inc + Dx * sin_inc) / sr0
+    return sr_surface, wave_dinc
+
+
 def pulse_scene(sc, snaps, p, rcs):
     """sr_surface, wave_dinc, sin_az and the per-pol complex scene at pulse p."""
     f = lerp_fields(snaps, sc, p, ('Dx', 'Dy', 'Dz'))
@@ -85,10 +94,8 @@ def pulse_scene(sc, snaps, p, rcs):
     sr = sc.sr.reshape(1, -1)
     sin_inc = sc.sin_inc.reshape(1, -1)
     cos_inc = sc.cos_inc.reshape(1, -1)
-    # sar_raw_nompi.py:366-369
-    sr_surface = (sr - cos_inc * f['Dz'] + az / 2 * sin_az + f['Dx'] * sin_inc +
-                  f['Dy'] * sin_az)
-    wave_dinc = (f['Dz'] * sin_inc + f['Dx'] * sin_inc) / sc.sr0
+    sr_surface, wave_dinc = facet_range(sr, cos_inc, sin_inc, az, sin_az, f['Dx'], f['Dy'],
+                                        f['Dz'], sc.sr0)
     scenes = {pol: np.zeros((sc.Ny, sc.Nx), dtype=np.complex128) for pol in sc.pols}
     k0 = sc.k0
     spec_energy = 0.

@@ -150,3 +150,22 @@ def test_arange_grid_detection():
     assert arange_grid(bent)[2] == 0
     assert arange_grid(np.linspace(1, 5, 9) ** 2)[2] == 0
     assert arange_grid(np.array([3.]))[2] == 0
+
+
+def test_frozen_ocean_holds_the_surface_at_t0(tmp_path):
+    """[ocean] frozen_ocean: every snapshot is the surface at t = 0 (the reference
+    skips the time update, sar_raw_nompi.py:357-358); the Bragg phase still
+    advances with the pulse index."""
+    from oceansar_amd import configs
+    import oracle.sar as osar
+    sc = configs.load_scene(configs.workload('small', L=64., n_pulses=128, K=4, frozen=True),
+                            tmp_path)
+    assert sc.frozen and all(sc.snapshot_time(m) == 0. for m in range(sc.n_snapshots))
+    cache = osar.SnapshotCache(sc)
+    a, b = cache.get(0), cache.get(7)
+    for n in a:
+        np.testing.assert_array_equal(a[n], b[n])
+    live = configs.load_scene(configs.workload('small', L=64., n_pulses=128, K=4), tmp_path)
+    assert not live.frozen and live.snapshot_time(7) == 28 * live.t_step
+    r0, _ = osar.simulate(sc, 40, 44, snaps=cache)
+    assert np.all(np.isfinite(r0)) and np.abs(r0).max() > 0
