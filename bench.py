@@ -4,10 +4,16 @@ A step is one full raw-data simulation of the workload: surface synthesis
 (FP64 FFTs) every K pulses, Bragg facet RCS at every refresh, and the fused
 echo accumulation of every facet into every pulse's range line for every
 channel. Inputs (wave coefficients, geometry tables, closure series) are
-resident in HBM before the timed region; the raw cube stays on the device.
+resident in HBM before the timed region; the raw cube stays on the device
+for `value`, and the same steps ending with the raw cube in pinned host
+memory give `value_with_host_copy`.
 
 Metric: facet-pulse contributions per second (one facet's echo deposited into
-one pulse's range line for one channel and polarisation).
+one pulse's range line for one channel and polarisation), and the p50/p99
+scene latency of the serving form (Poisson arrivals of mixed-size jobs),
+which the default run appends to the same line. The CPU baseline leg times
+the CPU restatement on bounded pulse samples and checks the GPU run's raw
+data on the same pulses against it (rel. RMS, phase RMS).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large]
 """
@@ -63,24 +69,77 @@ def measured_traffic(sc, shape_repr, pulses_per_launch):
     return None, None
 
 
-def cpu_baseline(workload, sample_pulses, threads):
-    """The C restatement of the reference loop (plus the numpy FP64 synthesis and
-    Bragg RCS it needs) timed on host cores over a bounded pulse sample."""
-    os.environ['OMP_NUM_THREADS'] = str(threads)
-    from oceansar_amd import configs
+def host_cores():
+    """(threads this process may use, os.cpu_count(), CPU model): the affinity
+    set, capped by a cgroup CPU quota when one is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else os.cpu_count()
+    try:
+        q, per = open('/sys/fs/cgroup/cpu.max').read().split()[:2]
+        if q != 'max':
+            n = min(n, max(1, int(float(q) / float(per) + 0.5)))
+    except (OSError, ValueError):
+        pass
+    model = ''
+    try:
+        for ln in open('/proc/cpuinfo'):
+            if ln.startswith('model name'):
+                model = ln.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return int(n), os.cpu_count(), model
+
+
+def cpu_baseline(sc, raw_gpu, sample_pulses, sample_pulses_1t):
+    """The CPU restatement of the reference loop on host cores, over bounded
+    pulse samples of the same scene: numpy FP64 surface synthesis of the
+    sample's snapshots and numpy FP64 Bragg refreshes (the reference's own
+    numpy stages), then the C pulse loop (the numba nopython stand-in) at
+    every available core and at one thread. The all-core sample is also the
+    accuracy check of the GPU run: the bench's raw cube on the same pulses
+    against this FP64 result."""
     import oracle.c_oracle as co
     import oracle.sar as osar
-    sc = configs.load_scene(configs.workload(workload), tempfile.mkdtemp())
-    p0 = sc.az_steps // 2
+    threads, n_os, model = host_cores()
+    K = sc.K
+    p0 = (sc.az_steps // 2 // K) * K - K // 2          # crosses a snapshot and a refresh
     p1 = p0 + sample_pulses
-    t = time.perf_counter()
-    co.simulate_c(sc, p0, p1, snaps=osar.SnapshotCache(sc))
-    dt = time.perf_counter() - t
-    w = sc.Nx * sc.Ny * (p1 - p0) * sc.num_ch * len(sc.pols)
-    return {'value': w / dt, 'unit': 'contributions/s', 'cores': threads, 'kind': 'port',
-            'sample': 'workload %s, pulses [%d, %d) of %d (%d snapshots, %d Bragg refreshes), '
-                      '%.1f s' % (workload, p0, p1, sc.az_steps, (p1 - 1) // sc.K - p0 // sc.K + 2,
-                                  len(set(sc.rcs_slot[p0:p1])), dt)}
+    n_snap = (p1 - 1) // K - p0 // K + 2
+    snaps = osar.SnapshotCache(sc, keep=n_snap + 2)
+    tim = {}
+    ref, _ = co.simulate_c(sc, p0, p1, snaps=snaps, threads=threads, timings=tim)
+    npol = len(sc.pols)
+    per_pulse = sc.Nx * sc.Ny * sc.num_ch * npol
+    w = per_pulse * (p1 - p0)
+    t_all = tim['synth_s'] + tim['bragg_s'] + tim['loop_s']
+    # one thread: the C loop only, over a shorter window of the same snapshots
+    tim1 = {}
+    co.simulate_c(sc, p0, p0 + sample_pulses_1t, snaps=snaps, threads=1, timings=tim1)
+    w1 = per_pulse * sample_pulses_1t
+    # accuracy of the GPU run on the sample pulses (noise-free, uncalibrated raw)
+    got = raw_gpu[:, :, p0:p1].cpu().numpy().astype(np.complex128)
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    mag = np.abs(ref)
+    keep = mag > mag.max() * 10 ** (-30. / 20.)
+    ph = float(np.sqrt(np.mean(np.angle(got[keep] * np.conj(ref[keep])) ** 2)))
+    n_refresh = len(set(int(r) for r in sc.rcs_slot[p0:p1]))
+    loop_1t = w1 / tim1['loop_s']
+    cpu = {
+        'value': w / t_all, 'unit': 'contributions/s', 'cores': threads, 'kind': 'port',
+        'sample': 'pulses [%d, %d) of %d: %d snapshots synthesised (numpy FP64), %d Bragg '
+                  'refreshes (numpy FP64), the C pulse loop at %d threads; %.1f s'
+                  % (p0, p1, sc.az_steps, n_snap, n_refresh, threads, t_all),
+        'stage_s': {k: round(v, 3) for k, v in tim.items()},
+        'loop_only_value': w / tim['loop_s'],
+        'loop_only_value_1_thread': loop_1t,
+        'sample_1_thread': 'pulses [%d, %d), C loop only, %.1f s'
+                           % (p0, p0 + sample_pulses_1t, tim1['loop_s']),
+        'host': {'threads_used': threads, 'os_cpu_count': n_os, 'cpu_model': model},
+    }
+    acc = {'rel_rms': err, 'phase_rms_rad': ph, 'against': 'C FP64 oracle (oracle/c), pulses '
+           '[%d, %d) of the timed run, %d channels' % (p0, p1, sc.num_ch),
+           'bound_rel_rms': 1e-4, 'bound_phase_rms_rad': 1e-3}
+    return cpu, acc
 
 
 def _scenes(kws, tag):
@@ -204,8 +263,13 @@ def main():
                          'the 2048^2 workload in one group, about 164 GB of fields, synthesis '
                          'workspace and amplitudes on the device; 128 / 256 measured 2.1 / '
                          '0.9 %% slower, with 41 / 82 GB)')
-    ap.add_argument('--cpu-sample', type=int, default=96)
+    ap.add_argument('--cpu-sample', type=int, default=96,
+                    help='pulses of the all-core CPU sample (also the accuracy window)')
+    ap.add_argument('--cpu-sample-1t', type=int, default=8,
+                    help='pulses of the one-thread CPU sample (C loop only)')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-serve', action='store_true',
+                    help='skip the serving run whose p50/p99 latencies join the default line')
     ap.add_argument('--overlap', action='store_true',
                     help='two streams: synthesis/Bragg of chunk c+1 beside the accumulation '
                          'of chunk c (measured no faster on the large workload: every stage '
@@ -244,13 +308,8 @@ def main():
     overlap = args.overlap
     # the launch group's fields, amplitudes and synthesis workspace must fit
     # the free device memory: halve it until they take at most 90 % of it
-    from oceansar_amd.batch import footprint_bytes, workspace_bytes
-    chunk = args.chunk
-    free = torch.cuda.mem_get_info()[0]
-    nbuf = 2 if overlap else 1
-    while chunk > 8 and (nbuf * footprint_bytes(sc, chunk) + workspace_bytes(sc, chunk)
-                         > 0.9 * free):
-        chunk //= 2
+    from oceansar_amd.batch import fit_chunk
+    chunk = fit_chunk(sc, args.chunk, torch.cuda.mem_get_info()[0], nbuf=2 if overlap else 1)
     for _ in range(args.warmup):
         ds.zero_c32(raw)
         ds.run(raw=raw, chunk_snaps=chunk, overlap=overlap)
@@ -268,6 +327,18 @@ def main():
         ds.run(raw=raw, chunk_snaps=chunk, timer=timer, overlap=overlap)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
+
+    # the same steps ending with the raw cube on the host (pinned memory), the
+    # scope "first launch to raw data on the host"
+    host_raw = torch.empty(raw.shape, dtype=raw.dtype, pin_memory=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ds.zero_c32(raw)
+        ds.run(raw=raw, chunk_snaps=chunk, overlap=overlap)
+        host_raw.copy_(raw, non_blocking=True)
+        torch.cuda.synchronize()
+    dt_host = (time.perf_counter() - t0) / args.steps
 
     per_stage = {}
     echo_ms = []
@@ -305,7 +376,11 @@ def main():
                    'stage_ms_per_step': {k: round(v, 3) for k, v in per_stage.items()},
                    'echo_launch_shape': repr(ds.shape),
                    'launch_group_intervals': chunk},
-        'roofline': {'bound': 'mfma', 'kernel': 'echo_accumulate_kernel',
+        'value_with_host_copy': W / dt_host,
+        'ms_per_step_with_host_copy': dt_host * 1e3,
+        'host_copy_note': 'the same steps, each ending with the %.0f MB raw cube copied to '
+                          'pinned host memory' % (raw.numel() * 8 / 1e6),
+        'roofline': {'bound': 'valu', 'kernel': 'echo_accumulate_kernel',
                      'achieved': achieved, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / FP32_PEAK_TFLOPS, 'traffic': traffic,
                      'traffic_source': traffic_src,
@@ -314,8 +389,8 @@ def main():
                      'flop_per_contribution': fpc,
                      'pulses_per_launch': pulses_per_launch,
                      'avg_launch_ms': float(np.mean(echo_ms)),
-                     'note': 'FP32 vector-ALU bound (VALU + transcendental + LDS atomics); '
-                             'peak is the MI355X FP32 figure'},
+                     'note': 'vector-ALU bound (FP32/FP64 VALU + transcendental + LDS atomics; '
+                             'no MFMA); peak is the MI355X FP32 vector figure'},
     }
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 4
@@ -326,8 +401,18 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        out['cpu_baseline'] = cpu_baseline(args.workload, args.cpu_sample, threads)
+        out['cpu_baseline'], out['accuracy'] = cpu_baseline(sc, raw, args.cpu_sample,
+                                                            args.cpu_sample_1t)
+    del ds, raw, host_raw
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    if not args.no_serve:
+        sv = bench_serve(args)
+        out['serving'] = {k: sv[k] for k in ('served', 'plain_sequential', 'p50_speedup_vs_plain',
+                                             'p99_speedup_vs_plain', 'throughput_vs_plain')}
+        out['serving']['workload'] = sv['config']['workload']
+        out['p50_latency_s'] = sv['served']['p50_latency_s']
+        out['p99_latency_s'] = sv['served']['p99_latency_s']
     print(json.dumps(out))
 
 

@@ -323,6 +323,17 @@ def footprint_bytes(sc, chunk=32):
     return int(tables + bufs + raw)
 
 
+def fit_chunk(sc, chunk, free_bytes, nbuf=1, frac=0.9):
+    """Largest launch group (snapshot intervals, halving from `chunk`, at least 8)
+    whose buffers (nbuf sets of chunk buffers, the resident tables, the raw cube
+    and the synthesis workspace) take at most `frac` of `free_bytes`."""
+    chunk = int(chunk)
+    while chunk > 8 and (nbuf * footprint_bytes(sc, chunk) + workspace_bytes(sc, chunk)
+                         > frac * free_bytes):
+        chunk //= 2
+    return chunk
+
+
 def workspace_bytes(sc, chunk=32):
     """Synthesis workspace of a context serving scenes of this size: one
     complex128 plane per packed spectrum and snapshot of a chunk, and the

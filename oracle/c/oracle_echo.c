@@ -16,6 +16,7 @@
  * in the numpy oracle's operation order; build with -ffp-contract=off.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -145,3 +146,9 @@ int oracle_echo(const oracle_desc* d) {
 }
 
 int64_t oracle_desc_size(void) { return (int64_t)sizeof(oracle_desc); }
+
+/* OpenMP threads of the pulse loop (the environment is read only once). */
+int oracle_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}

@@ -47,11 +47,13 @@ _lib = None
 def load():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
             build()
         lib = ctypes.CDLL(LIB)
         lib.oracle_echo.argtypes = [ctypes.POINTER(Desc)]
         lib.oracle_desc_size.restype = c_int64
+        lib.oracle_set_threads.argtypes = [ctypes.c_int]
+        lib.oracle_set_threads.restype = ctypes.c_int
         if lib.oracle_desc_size() != ctypes.sizeof(Desc):
             raise ImportError('oracle_desc layout mismatch')
         _lib = lib
@@ -62,18 +64,30 @@ def _p(a):
     return a.ctypes.data_as(c_void_p)
 
 
-def simulate_c(sc, p0=0, p1=None, snaps=None, threads=None):
-    """C-oracle raw cube for pulses [p0, p1) (un-calibrated, noise-free) and NRCS_avg."""
-    if threads is not None:
-        os.environ['OMP_NUM_THREADS'] = str(threads)
+def set_threads(n):
+    """OpenMP threads of the C pulse loop; returns the count in force."""
+    return load().oracle_set_threads(int(n))
+
+
+def simulate_c(sc, p0=0, p1=None, snaps=None, threads=None, timings=None):
+    """C-oracle raw cube for pulses [p0, p1) (un-calibrated, noise-free) and NRCS_avg.
+
+    timings: optional dict that receives the seconds spent in the snapshot
+    synthesis ('synth_s', numpy FP64, only for snapshots not yet cached), the
+    Bragg RCS refreshes ('bragg_s', numpy FP64) and the C pulse loop ('loop_s')."""
+    import time
     lib = load()
+    if threads is not None:
+        lib.oracle_set_threads(int(threads))
     p1 = sc.az_steps if p1 is None else p1
     snaps = snaps or osar.SnapshotCache(sc)
     K = sc.K
     m0 = p0 // K
     last = p1 - 1
     m1 = last // K + (1 if last % K else 0)
+    t0 = time.perf_counter()
     S = [snaps.get(m) for m in range(m0, m1 + 1)]
+    t_synth = time.perf_counter() - t0
     keep = []
 
     def arr(x, dt):
@@ -103,6 +117,7 @@ def simulate_c(sc, p0=0, p1=None, snaps=None, threads=None):
     d.sinc = _p(arr(sc.sinc_vec, np.float32))
     out = arr(np.zeros((npol, sc.num_ch, p1 - p0, sc.rg_samp, 2)), np.float64)
     d.out = _p(out)
+    t0 = time.perf_counter()
     if sc.bragg_enable:
         d.bragg = 1
         r0, r1 = int(sc.rcs_slot[p0]), int(sc.rcs_slot[last])
@@ -122,6 +137,11 @@ def simulate_c(sc, p0=0, p1=None, snaps=None, threads=None):
         d.nm, d.np_ = sc.scat_m.N, sc.scat_p.N
         d.ind0_m = _p(arr(sc.scat_m.ind0, np.int64))
         d.ind0_p = _p(arr(sc.scat_p.ind0, np.int64))
+    t_bragg = time.perf_counter() - t0
+    t0 = time.perf_counter()
     lib.oracle_echo(ctypes.byref(d))
+    t_loop = time.perf_counter() - t0
+    if timings is not None:
+        timings.update(synth_s=t_synth, bragg_s=t_bragg, loop_s=t_loop)
     raw = out[..., 0] + 1j * out[..., 1]
     return raw, nrcs

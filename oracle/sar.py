@@ -20,8 +20,9 @@ from . import surface as _surface
 class SnapshotCache:
     """Surface snapshots (float32 fields) synthesised on demand at t = m*K*t_step."""
 
-    def __init__(self, sc, compute=None):
+    def __init__(self, sc, compute=None, keep=4):
         self.sc = sc
+        self.keep = int(keep)
         if compute is None:
             compute = ('D', 'Diff', 'Diff2') if sc.spec_enable else ('D', 'Diff')
         self.compute = compute
@@ -29,7 +30,7 @@ class SnapshotCache:
 
     def get(self, m):
         if m not in self.cache:
-            if len(self.cache) > 3:
+            if len(self.cache) >= self.keep:
                 self.cache.pop(min(self.cache))
             # a frozen ocean stays at t = 0 (sar_raw_nompi.py:357-358 skips the
             # time update; the build evaluates the surface once at t = 0)

@@ -53,19 +53,28 @@ def test_large_workload_window_matches_c_oracle(need_gpu, large_scene):
 
 def test_full_run_reproduces_window(need_gpu, large_scene):
     """Every pulse's echo depends only on that pulse: the whole 8192-pulse run
-    in the bench's launch groups (128 snapshot intervals) gives the window's
-    raw data and NRCS of a separate 40-pulse run in 2-interval groups (to the
-    float32 rounding of the atomic accumulation order), so the oracle check of
-    the window extends to the full run."""
+    in the launch group bench.py picks (512 snapshot intervals, the whole run,
+    unless the free device memory forces fewer: field buffers of 2.15e9
+    elements, past 32-bit element indices) gives the window's raw data and
+    NRCS of a separate 40-pulse run in 2-interval groups (to the float32
+    rounding of the atomic accumulation order), so the oracle check of the
+    window extends to the full run at the bench's launch shape."""
     import torch
+    from oceansar_amd.batch import fit_chunk
     from oceansar_amd.engine import DeviceScene
     sc = large_scene
     ds = DeviceScene(sc)
     p0, p1 = 4088, 4128
     win, win_nrcs = ds.run(p0, p1, chunk_snaps=2)
     win = win[:, :, p0:p1].cpu().numpy()
-    full, full_nrcs = ds.run(chunk_snaps=128)
     torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    chunk = fit_chunk(sc, 512, torch.cuda.mem_get_info()[0])
+    print('full run in launch groups of %d snapshot intervals' % chunk)
+    full, full_nrcs = ds.run(chunk_snaps=chunk)
+    torch.cuda.synchronize()
+    if chunk == 512:
+        assert (chunk + 1) * sc.Nx * sc.Ny > 2 ** 31
     assert rel_rms(full[:, :, p0:p1].cpu().numpy(), win) < 1e-6
     assert np.allclose(full_nrcs[:, p0:p1], win_nrcs[:, p0:p1], rtol=1e-12)
     assert np.isfinite(full_nrcs).all() and (full_nrcs > 0).all()
