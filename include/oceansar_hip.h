@@ -104,9 +104,7 @@ typedef struct osar_echo_desc {
   const double* sin_inc;
   const double* cos_inc;
   const double* dinc_col;   /* inc - inc_angle */
-  const osar_c32* cap_tab;  /* Bragg phase table [nx][n_pulses_total]: (cos, sin) of
-                               cap_rate[x] (p + 1), cap_rate = 2 k0 t_step c_b
-                               (sar_raw_nompi.py:470-480); osar_echo_cap_table fills it */
+  const double* cap_rate;   /* 2 k0 t_step c_b; Bragg phase is cap_rate * (p + 1) */
   /* per-row FP64 table [ny] */
   const double* y_row;
   /* surface snapshots, float32 [n_snap_slots][nx][ny] (x-major) */
@@ -140,15 +138,6 @@ typedef struct osar_echo_desc {
 } osar_echo_desc;
 
 int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void* stream);
-
-/* The Bragg phase table of osar_echo_desc.cap_tab: out[x][p] = (cos, sin) of
-   cap_rate[x] (p + 1) for x < nx, p < n_pulses (cap_rate float64 [nx], out
-   complex64 [nx][n_pulses]). The phase the reference adds per pulse to the two
-   Bragg components, -/+ 2 k0 t_step c_b (az_step + 1) (sar_raw_nompi.py:470-480),
-   is a function of the column and the pulse only, so it is tabulated once per
-   scene; the accumulator then needs one sin/cos per facet and pulse. */
-int osar_echo_cap_table(osar_ctx* ctx, const double* cap_rate, int32_t nx, int32_t n_pulses,
-                        osar_c32* out, void* stream);
 
 /* Zero-fill n complex64 values (a kernel, so it may be captured in a graph). */
 int osar_fill_zero_c32(osar_ctx* ctx, osar_c32* p, int64_t n, void* stream);

@@ -35,7 +35,7 @@ class EchoDesc(ctypes.Structure):
             'ltx_over_l', 'lrx_over_l', 'sr_col_min', 'sr_col_max', 'resid_max')] + \
         [('b_ati', c_double * MAX_CH), ('b_xti', c_double * MAX_CH)] + \
         [(n, c_void_p) for n in (
-            'sr_col', 'sin_inc', 'cos_inc', 'dinc_col', 'cap_tab', 'y_row', 'dx', 'dy', 'dz',
+            'sr_col', 'sin_inc', 'cos_inc', 'dinc_col', 'cap_rate', 'y_row', 'dx', 'dy', 'dz',
             'amp', 'rcs_slot', 'scat_m', 'scat_p', 'ind0_m', 'ind0_p', 'sinc_tab', 'raw',
             'sx', 'sy', 'sxx', 'syy', 'sxy', 'inc_col')] + \
         [('facet_dx', c_double), ('facet_dy', c_double), ('spec_energy', c_void_p)]
@@ -67,7 +67,7 @@ FIELD_ORDER = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
 
 EXPORTS = ['osar_ctx_create', 'osar_ctx_destroy', 'osar_last_error', 'osar_version',
            'osar_struct_size', 'osar_echo_accumulate', 'osar_fill_zero_c32',
-           'osar_echo_deposit', 'osar_echo_cap_table', 'osar_facet_bragg', 'osar_fill_zero_f64',
+           'osar_echo_deposit', 'osar_facet_bragg', 'osar_fill_zero_f64',
            'osar_surface_workspace_bytes', 'osar_surface_synth']
 
 _lib = None
@@ -93,8 +93,6 @@ def load():
     lib.osar_fill_zero_f64.argtypes = [c_void_p, c_void_p, c_int64, c_void_p]
     lib.osar_echo_deposit.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_double, c_double,
                                       c_void_p, c_int32, c_int32, c_void_p, c_int32, c_void_p]
-    lib.osar_echo_cap_table.argtypes = [c_void_p, c_void_p, c_int32, c_int32, c_void_p,
-                                        c_void_p]
     lib.osar_facet_bragg.argtypes = [c_void_p, POINTER(BraggDesc), c_void_p]
     lib.osar_surface_workspace_bytes.argtypes = [POINTER(SurfaceDesc)]
     lib.osar_surface_workspace_bytes.restype = c_int64

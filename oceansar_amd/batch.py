@@ -316,7 +316,6 @@ def footprint_bytes(sc, chunk=32):
         tables += 8 * (sc.scat_m.N + sc.scat_p.N + 2 * sc.az_steps)    # extended series
         tables += 4 * sc.az_steps + 8 * len(sc.rcs_refresh)            # schedules
         tables += 16 * len(sc.bragg.spec_k)                            # spectrum table
-        tables += 8 * sc.Nx * sc.az_steps                              # Bragg phase table
     bufs = (chunk + 1) * n2 * 4 * len(names)
     if sc.bragg_enable:
         bufs += (chunk + 2) * npol * 2 * n2 * 4

@@ -71,13 +71,6 @@ __device__ __forceinline__ float2 cis_fp64(double phase) {
   return make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
 }
 
-// exp(i 2 pi r) for a phase r in revolutions (FP64): the integer part is
-// removed in FP64, the fraction in [-1/2, 1/2] goes through the hardware sin/cos.
-__device__ __forceinline__ float2 cis_rev(double r) {
-  const float f = (float)(r - rint(r));
-  return make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
-}
-
 // a / b for b > 0 given inv = 1 / b correctly rounded: one residual correction
 // of a * inv gives the correctly rounded quotient (Markstein), i.e. the same
 // bits as IEEE division, at three FP64 operations.

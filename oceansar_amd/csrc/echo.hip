@@ -238,7 +238,6 @@ struct FacetIn {
   float dx0, dy0, dz0, dx1, dy1, dz1;
   float amp[NPOL][2];
   osar_c32 sm, sp;
-  osar_c32 cap;        // (cos, sin) of the Bragg phase advance cap_rate[x] (p + 1)
   float s0[5], s1[5];  // specular slope / curvature fields at m and m+1
 };
 
@@ -261,13 +260,10 @@ __device__ __forceinline__ void load_fields(const FacetBases<NPOL>& B, uint32_t 
 
 // bragg: the call's Bragg flag (uniform). Lanes whose pulse has no RCS refresh
 // in the call's slots read slot 0 and do not use the values.
-// cap8: byte offset of the column's entry for this lane's pulse in the Bragg
-// phase table cap_tab[nx][n_pulses_total] (below 2^32, checked on the host);
-// the lanes of one pulse group read one 64-byte run per column.
 template <int NPOL, bool SPEC>
 __device__ __forceinline__ void load_scat(const osar_echo_desc& d, const FacetBases<NPOL>& B,
                                           uint32_t idx4, uint32_t f0, uint32_t f1, int p,
-                                          bool bragg, const FacetIdx& fi, uint32_t cap8,
+                                          bool bragg, const FacetIdx& fi,
                                           FacetIn<NPOL, SPEC>& f) {
   if (bragg) {
 #pragma unroll
@@ -278,7 +274,6 @@ __device__ __forceinline__ void load_scat(const osar_echo_desc& d, const FacetBa
     // byte offsets below 2^32: the host bounds the series to 2^29 entries
     f.sm = load_at(d.scat_m, (uint32_t)(fi.m + p) << 3);
     f.sp = load_at(d.scat_p, (uint32_t)(fi.p + p) << 3);
-    f.cap = load_at(d.cap_tab, cap8);
   }
   if (SPEC) {
     const uint32_t o0 = idx4 + f0, o1 = idx4 + f1;
@@ -472,7 +467,7 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
   L.nbins = d.nbins;
   L.npt = d.n_pulses_total;
   L.raw = d.raw;
-  const double m2k0_rev = -(2 * d.k0) * osar::kInvTwoPi;   // surface phase in revolutions per m
+  const double two_k0 = 2 * d.k0;
   const double inv_binsize = 1.0 / d.binsize;
   const int spol = NPOL == 2 ? 2 : (d.pol == 1 ? 1 : 0);
   double spec_acc = 0.;
@@ -511,15 +506,12 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
     FacetIdx ia = {0, 0};
     const uint32_t col4 = (uint32_t)d.ny * 4u;   // bytes between columns of a field plane
     const uint32_t idx4_lo = ((uint32_t)x_lo * d.ny + yrow) * 4u;
-    const uint32_t capcol8 = (uint32_t)d.n_pulses_total * 8u;   // bytes between columns
-    const uint32_t cap8_lo = (uint32_t)x_lo * capcol8 + (uint32_t)p * 8u;
     load_index(d, idx4_lo, bragg, ia);
     load_fields<NPOL, SPEC>(B, idx4_lo, f0, f1, cur);
-    load_scat<NPOL, SPEC>(d, B, idx4_lo, f0, f1, p, bragg, ia, cap8_lo, cur);
+    load_scat<NPOL, SPEC>(d, B, idx4_lo, f0, f1, p, bragg, ia, cur);
     load_index(d, idx4_lo + (uint32_t)(min(x_lo + 1, x_hi - 1) - x_lo) * col4, bragg, ia);
     for (int x = x_lo; x < x_hi; ++x) {
-      const uint32_t xn = (uint32_t)(min(x + 1, x_hi - 1) - x_lo);
-      const uint32_t idx4n = idx4_lo + xn * col4;
+      const uint32_t idx4n = idx4_lo + (uint32_t)(min(x + 1, x_hi - 1) - x_lo) * col4;
       const uint32_t idx4nn = idx4_lo + (uint32_t)(min(x + 2, x_hi - 1) - x_lo) * col4;
       const double Dx = lerp2(cur.dx0, cur.dx1, rk, wl, wh);
       const double Dy = lerp2(cur.dy0, cur.dy1, rk, wl, wh);
@@ -539,21 +531,16 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
 #pragma unroll
       for (int pol = 0; pol < NPOL; ++pol) scene[pol] = make_float2(0.f, 0.f);
       if (bragg_ok) {
-        // a0 s_m e^{i(surf - cap)} + a1 s_p e^{i(surf + cap)} = e^{i surf} X with
-        // X = A0 conj(c) + A1 c, A0 = a0 s_m, A1 = a1 s_p, c = e^{i cap} from the
-        // phase table: one FP32 sin/cos pair per facet and pulse (the surface
-        // phase, reduced to revolutions in FP64) instead of two
-        const float2 es = osar::cis_rev(g.srs * m2k0_rev);
-        const float cc = cur.cap.re, cs = cur.cap.im;
+        const double surf = -two_k0 * g.srs;
+        const double cap = uniform_load(d.cap_rate, x) * (double)(p + 1);
+        const float2 e0 = osar::cis_fp64(surf - cap);
+        const float2 e1 = osar::cis_fp64(surf + cap);
+        const float2 b0 = osar::cmul(e0, make_float2(cur.sm.re, cur.sm.im));
+        const float2 b1 = osar::cmul(e1, make_float2(cur.sp.re, cur.sp.im));
 #pragma unroll
         for (int pol = 0; pol < NPOL; ++pol) {
           const float a0 = cur.amp[pol][0], a1 = cur.amp[pol][1];
-          const float2 A0 = make_float2(a0 * cur.sm.re, a0 * cur.sm.im);
-          const float2 A1 = make_float2(a1 * cur.sp.re, a1 * cur.sp.im);
-          const float2 S = make_float2(A0.x + A1.x, A0.y + A1.y);
-          const float2 D = make_float2(A1.x - A0.x, A1.y - A0.y);
-          const float2 X = make_float2(cc * S.x - cs * D.y, cc * S.y + cs * D.x);
-          scene[pol] = osar::cmul(es, X);
+          scene[pol] = make_float2(a0 * b0.x + a1 * b1.x, a0 * b0.y + a1 * b1.y);
         }
       }
       if (SPEC) {
@@ -576,7 +563,7 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
           }
         }
       }
-      load_scat<NPOL, SPEC>(d, B, idx4n, f0, f1, p, bragg, ia, cap8_lo + xn * capcol8, cur);
+      load_scat<NPOL, SPEC>(d, B, idx4n, f0, f1, p, bragg, ia, cur);
       load_index(d, idx4nn, bragg, ia);
       if (finite) {
         if (!XTI) {
@@ -765,34 +752,7 @@ __global__ void echo_deposit_kernel(const double* srg, const osar_c32* scene, in
   }
 }
 
-// Bragg phase table: tab[x][p] = (cos, sin)(cap_rate[x] (p + 1)), the phase
-// reduced to revolutions in FP64 before the FP32 sin/cos (cis_fp64).
-__global__ void echo_cap_table_kernel(const double* cap_rate, int nx, int npt, osar_c32* tab) {
-  const int64_t n = (int64_t)nx * npt;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int x = (int)(i / npt), p = (int)(i - (int64_t)x * npt);
-    const float2 c = osar::cis_fp64(cap_rate[x] * (double)(p + 1));
-    tab[i] = osar_c32{c.x, c.y};
-  }
-}
-
 }  // namespace
-
-extern "C" int osar_echo_cap_table(osar_ctx* ctx, const double* cap_rate, int32_t nx,
-                                   int32_t n_pulses, osar_c32* out, void* stream) {
-  OSAR_PTR(ctx);
-  OSAR_REQUIRE(nx > 0 && n_pulses > 0, OSAR_EINVAL, "empty Bragg phase table");
-  OSAR_PTR(cap_rate);
-  OSAR_PTR(out);
-  OSAR_HIP(hipSetDevice(ctx->device));
-  const int64_t n = (int64_t)nx * n_pulses;
-  const int64_t blocks = (n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192;
-  hipLaunchKernelGGL(echo_cap_table_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                     (hipStream_t)stream, cap_rate, nx, n_pulses, out);
-  OSAR_LAUNCHED("echo_cap_table_kernel");
-  return OSAR_OK;
-}
 
 extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void* stream) {
   OSAR_PTR(ctx);
@@ -830,15 +790,11 @@ extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void
     OSAR_PTR(d->scat_p);
     OSAR_PTR(d->ind0_m);
     OSAR_PTR(d->ind0_p);
-    OSAR_PTR(d->cap_tab);
+    OSAR_PTR(d->cap_rate);
     // scatterer values are gathered at 32-bit byte offsets
     OSAR_REQUIRE(d->scat_len > d->n_pulses_total && d->scat_len <= (1 << 29), OSAR_EINVAL,
                  "scat_len %d: the scatterer series need n_pulses_total < entries <= 2^29",
                  d->scat_len);
-    // and so is the Bragg phase table
-    OSAR_REQUIRE((uint64_t)d->nx * d->n_pulses_total * 8 < (1ull << 32), OSAR_EUNSUPPORTED,
-                 "Bragg phase table of %d columns x %d pulses too large for 32-bit offsets",
-                 d->nx, d->n_pulses_total);
   }
   if (d->flags & OSAR_ECHO_XTI) OSAR_PTR(d->dinc_col);
   if (d->flags & OSAR_ECHO_SPECULAR) {

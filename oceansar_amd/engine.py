@@ -143,12 +143,6 @@ class DeviceScene:
         self.n_pulses = sc.az_steps
         if sc.bragg_enable:
             self.cap_rate = _t(sc.cap_rate, f64, dev)
-            # Bragg phase advance per column and pulse, tabulated once on the device
-            self.cap_tab = torch.empty((sc.Nx, self.n_pulses), dtype=c64, device=dev)
-            _lib.check(self.ctx.lib.osar_echo_cap_table(self.ctx.h, _lib.ptr(self.cap_rate),
-                                                         sc.Nx, self.n_pulses,
-                                                         _lib.ptr(self.cap_tab),
-                                                         _lib.stream_ptr()))
             self.scat_m = _t(self._extend(sc.scat_m), c64, dev)
             self.scat_p = _t(self._extend(sc.scat_p), c64, dev)
             # x-major [nx][ny], like every per-facet device array
@@ -367,7 +361,7 @@ class DeviceScene:
         d.sinc_tab = _lib.ptr(self.sinc)
         d.raw = _lib.ptr(raw)
         if sc.bragg_enable:
-            d.cap_tab = _lib.ptr(self.cap_tab)
+            d.cap_rate = _lib.ptr(self.cap_rate)
             d.amp = _lib.ptr(amp)
             d.rcs_slot = _lib.ptr(self.rcs_slot)
             d.scat_m, d.scat_p = _lib.ptr(self.scat_m), _lib.ptr(self.scat_p)

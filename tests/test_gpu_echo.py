@@ -187,18 +187,3 @@ def test_deposit_matches_oracle(need_gpu):
                  1.3, -2.73, torch.as_tensor(tab, device='cuda'), 8, 20, out)
     got = out.cpu().numpy()
     assert rel_rms(got, ref) < 1e-5
-
-
-def test_bragg_phase_table(need_gpu, cfg1_scene):
-    """osar_echo_cap_table: (cos, sin) of cap_rate[x] (p + 1) for every column and
-    pulse, the phase advance of the two Bragg components (sar_raw_nompi.py:470-480),
-    reduced to revolutions in FP64 before the FP32 sin/cos (abs error ~1e-7)."""
-    import torch
-    from oceansar_amd.engine import DeviceScene
-    sc = cfg1_scene
-    ds = DeviceScene(sc)
-    torch.cuda.synchronize()
-    got = ds.cap_tab.cpu().numpy().astype(np.complex128)
-    assert got.shape == (sc.Nx, sc.az_steps)
-    ref = np.exp(1j * sc.cap_rate[:, None] * (np.arange(sc.az_steps)[None, :] + 1.))
-    assert np.max(np.abs(got - ref)) < 3e-7
