@@ -218,6 +218,13 @@ typedef struct osar_surface_desc {
                                kx[nx - i] = -kx[i] for i != nx / 2, the same for
                                ky): selects the faster row pass; 0 gives the
                                same results for any tables */
+  int32_t slope_fp32;       /* 1: Diffx and Diffy share one spectrum whose row-pass
+                               result is stored as complex64 (every other field, and
+                               both slopes when choppy, round-trip in complex128):
+                               5 fields take 40 instead of 48 workspace bytes per grid
+                               point; the slopes then carry the float32 rounding of
+                               the intermediate (about 1e-7 of their RMS) */
+  int32_t reserved;
   double depth;             /* water depth, <= 0 for deep water */
   double current_x, current_y;
   double kx_res, ky_res;    /* float32 grid steps (good-k threshold) */

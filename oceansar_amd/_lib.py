@@ -56,7 +56,7 @@ class BraggDesc(ctypes.Structure):
 
 class SurfaceDesc(ctypes.Structure):
     _fields_ = [(n, c_int32) for n in ('nx', 'ny', 'fields', 'choppy', 'n_times',
-                                       'mirror_grid')] + \
+                                       'mirror_grid', 'slope_fp32', 'reserved')] + \
         [(n, c_double) for n in ('depth', 'current_x', 'current_y', 'kx_res', 'ky_res')] + \
         [(n, c_void_p) for n in ('kx', 'ky', 'coefs', 'times')] + [('out', c_void_p * 8)]
 

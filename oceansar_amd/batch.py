@@ -335,11 +335,15 @@ def fit_chunk(sc, chunk, free_bytes, nbuf=1, frac=0.9):
 
 
 def workspace_bytes(sc, chunk=32):
-    """Synthesis workspace of a context serving scenes of this size: one
-    complex128 plane per packed spectrum and snapshot of a chunk, and the
-    float64 dispersion table (osar_surface_workspace_bytes)."""
-    nf = 3 + (2 if sc.bragg_enable else 0) + (5 if sc.spec_enable else 0)
-    if sc.spec_enable and sc.bragg_enable:
-        nf = 8
-    nq = (nf + 1) // 2
-    return int(nq * (chunk + 1) * sc.Nx * sc.Ny * 16 + sc.Nx * sc.Ny * 8)
+    """Synthesis workspace of a context serving scenes of this size: per
+    snapshot of a chunk one complex128 plane per packed spectrum (and, for
+    Bragg-only scenes, one complex64 plane of the two slopes), and the float64
+    dispersion table (osar_surface_workspace_bytes)."""
+    n2 = sc.Nx * sc.Ny
+    if sc.spec_enable:
+        per = 16 * 4                                   # 8 fields: 4 complex128 spectra
+    elif sc.bragg_enable:
+        per = 16 * 2 + 8                               # (Dz, Dx), (Dy), slopes in complex64
+    else:
+        per = 16 * 2                                   # (Dz, Dx), (Dy)
+    return int(per * (chunk + 1) * n2 + 8 * n2)

@@ -54,6 +54,9 @@ struct QPlan {
   int outa, outb;  // output field index (0..7) or -1
   int choppy;      // 1: single output = Re / (1 + Im) into outa
   int ca, cb;      // the multipliers as monomial codes (mono_code)
+  int f32;         // 1: the row pass stores this spectrum as complex64 (else complex128)
+  int cbw, lcbw;   // columns per contiguous run of a workspace row (128 bytes) and log2
+  long long off;   // byte offset of this spectrum's plane within a snapshot's workspace
 };
 
 // Every multiplier is a monomial s i^d kx^px ky^py |k|^-pk. Code bits: px (0-1),
@@ -62,7 +65,6 @@ constexpr int kMonoImag = 32, kMonoNeg = 64, kMonoNone = 128;
 
 struct SynthArgs {
   int nx, ny, nq, logx, logy, cb, lcb, n_times;
-  int cbw, lcbw;   // workspace block width (columns per contiguous run of a row) and log2
   double depth, cur_x, cur_y, thr;
   const float* kx;
   const float* ky;
@@ -70,7 +72,8 @@ struct SynthArgs {
   const double* times;
   const double2* twx;   // [128]: exp(2 pi i k / nx) for k < 64, then for k = 64 m
   const double2* twy;
-  double2* work;
+  unsigned char* work;  // workspace: per snapshot, the planes of the nq spectra
+  long long snap_bytes; // workspace bytes per snapshot
   double* omega;        // [ny][nx] omega(|k|), filled by omega_kernel
   float* out[8];
   QPlan q[kMaxQ];
@@ -120,6 +123,20 @@ __device__ __forceinline__ void store_stream(double2* p, double2 v) {
 __device__ __forceinline__ double2 load_stream(const double2* p) {
   const nt_double2 x = __builtin_nontemporal_load((const nt_double2*)p);
   return make_double2(x.x, x.y);
+}
+
+// complex64 workspace elements: the FP64 value rounded to float32 parts
+typedef float nt_float2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_stream32(float2* p, double2 v) {
+  nt_float2 x;
+  x.x = (float)v.x;
+  x.y = (float)v.y;
+  __builtin_nontemporal_store(x, (nt_float2*)p);
+}
+
+__device__ __forceinline__ double2 load_stream32(const float2* p) {
+  const nt_float2 x = __builtin_nontemporal_load((const nt_float2*)p);
+  return make_double2((double)x.x, (double)x.y);
 }
 
 __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
@@ -458,16 +475,18 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
   // Row pairs in XCD-contiguous order: rows j and j + 1 fill one 128-byte
   // line of the workspace between them, so their workgroups should run
   // behind the same L2.
+  // Items are row pair major, snapshot minor: the snapshots of a row pair run
+  // one after another on one XCD, so its coefficients and dispersion values
+  // come from HBM once and from that XCD's L2 for the other snapshots.
   const int npairs = a.ny / 2 + 1;
   const int item = osar::xcd_item(blockIdx.x, npairs * a.n_times);
-  const int j = item % npairs;
+  const int ti = item % a.n_times;
+  const int j = item / a.n_times;
   const int jm = (a.ny - j) & (a.ny - 1);
   const int nrow = (j == jm) ? 1 : 2;
-  const int ti = item / npairs;
   const double t = a.times[ti];
   const int tid = threadIdx.x;
   const int nx = a.nx;
-  const int cbw = a.cbw;
   const float kyj = a.ky[j], kyjm = a.ky[jm];
   double2 H[kRowPairs];    // H(a) at (j, i)
   // Every element of the register arrays is assigned on every path (columns
@@ -573,12 +592,25 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
     }
     __syncthreads();
     lds_ifft<kRowThreads, 2 * kMaxN / kRowThreads, 3, SHORT>(S, nx, logx, nrow, nx, tw);
-    double2* w = a.work + (size_t)(ti * a.nq + qi) * a.ny * nx;
-    for (int e = tq; e < nrow * nx; e += kRowThreads) {
-      const int r = e >> logx, i = e & (nx - 1);
-      const int row = r == 0 ? j : jm;
-      const int xb = i >> a.lcbw, c = i & (cbw - 1);
-      store_stream(&w[((size_t)xb * a.ny + row) * cbw + c], S[pad(e)]);
+    // column-blocked plane W[x / cbw][y][x % cbw] of this spectrum and snapshot
+    unsigned char* wp = a.work + (size_t)ti * (size_t)a.snap_bytes + (size_t)q.off;
+    const int cbw = q.cbw, lcbw = q.lcbw;
+    if (q.f32) {
+      float2* w = (float2*)wp;
+      for (int e = tq; e < nrow * nx; e += kRowThreads) {
+        const int r = e >> logx, i = e & (nx - 1);
+        const int row = r == 0 ? j : jm;
+        const int xb = i >> lcbw, c = i & (cbw - 1);
+        store_stream32(&w[((size_t)xb * a.ny + row) * cbw + c], S[pad(e)]);
+      }
+    } else {
+      double2* w = (double2*)wp;
+      for (int e = tq; e < nrow * nx; e += kRowThreads) {
+        const int r = e >> logx, i = e & (nx - 1);
+        const int row = r == 0 ? j : jm;
+        const int xb = i >> lcbw, c = i & (cbw - 1);
+        store_stream(&w[((size_t)xb * a.ny + row) * cbw + c], S[pad(e)]);
+      }
     }
     __syncthreads();
   }
@@ -591,7 +623,7 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
 template <bool SHORT>
 __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  const int cb = a.cb, cbw = a.cbw;
+  const int cb = a.cb;
   double2* C = (double2*)lds_raw;  // [cb][ny]
   double2* tw = C + padded((size_t)a.ny * cb);  // [128] twiddles
   load_twiddles(tw, a.twy);
@@ -603,15 +635,27 @@ __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(Synt
   const int x0 = xb * cb;
   const int tid = threadIdx.x;
   const int ny = a.ny, nx = a.nx;
-  const double2* w = a.work + (size_t)(ti * a.nq + qi) * ny * nx +
-                     (size_t)(x0 >> a.lcbw) * ny * cbw + (x0 & (cbw - 1));
+  const QPlan q = plan_at(a, qi);
+  const int cbw = q.cbw;
+  const size_t wofs = (size_t)(x0 >> q.lcbw) * ny * cbw + (x0 & (cbw - 1));
+  const unsigned char* wp = a.work + (size_t)ti * (size_t)a.snap_bytes + (size_t)q.off;
   // every load of the block in flight before the first LDS store
   constexpr int kFill = kColElems / kColThreads;
   double2 fill[kFill];
+  if (q.f32) {
+    const float2* w = (const float2*)wp + wofs;
 #pragma unroll
-  for (int k = 0; k < kFill; ++k) {
-    const int e = tid + k * kColThreads;
-    if (e < ny * cb) fill[k] = load_stream(&w[(size_t)(e >> a.lcb) * cbw + (e & (cb - 1))]);
+    for (int k = 0; k < kFill; ++k) {
+      const int e = tid + k * kColThreads;
+      if (e < ny * cb) fill[k] = load_stream32(&w[(size_t)(e >> a.lcb) * cbw + (e & (cb - 1))]);
+    }
+  } else {
+    const double2* w = (const double2*)wp + wofs;
+#pragma unroll
+    for (int k = 0; k < kFill; ++k) {
+      const int e = tid + k * kColThreads;
+      if (e < ny * cb) fill[k] = load_stream(&w[(size_t)(e >> a.lcb) * cbw + (e & (cb - 1))]);
+    }
   }
 #pragma unroll
   for (int k = 0; k < kFill; ++k) {
@@ -620,7 +664,6 @@ __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(Synt
   }
   __syncthreads();
   lds_ifft<kColThreads, kColElems / kColThreads, kColRadixLog, SHORT>(C, ny, a.logy, cb, ny, tw);
-  const QPlan q = plan_at(a, qi);
   const double scale = 1.0 / ((double)nx * (double)ny);
   const double t = a.times[ti];
   const size_t plane = (size_t)nx * ny;
@@ -695,10 +738,23 @@ int make_plan(const osar_surface_desc* d, QPlan* q, int* nq) {
     }
     items.push_back(Item{f, kMult[f]});
   }
+  // slope_fp32: Diffx and Diffy share one complex64 spectrum, the other fields
+  // pair up in complex128 spectra
+  std::vector<Item> lo;
+  if (d->slope_fp32 && !d->choppy) {
+    std::vector<Item> keep;
+    for (const Item& it : items) (it.out == 3 || it.out == 4 ? lo : keep).push_back(it);
+    items.swap(keep);
+  }
   for (size_t i = 0; i < items.size(); i += 2) {
     const bool two = i + 1 < items.size();
     q[n++] = QPlan{items[i].m, two ? items[i + 1].m : (int)M_NONE, items[i].out,
-                   two ? items[i + 1].out : -1, 0, 0, 0};
+                   two ? items[i + 1].out : -1, 0, 0, 0, 0, 0, 0, 0};
+  }
+  if (!lo.empty()) {
+    const bool two = lo.size() > 1;
+    q[n++] = QPlan{lo[0].m, two ? lo[1].m : (int)M_NONE, lo[0].out, two ? lo[1].out : -1, 0,
+                   0, 0, 1, 0, 0, 0};
   }
   for (int i = 0; i < n; ++i) {
     q[i].ca = mono_code(q[i].ma);
@@ -706,6 +762,32 @@ int make_plan(const osar_surface_desc* d, QPlan* q, int* nq) {
   }
   *nq = n;
   return OSAR_OK;
+}
+
+// Column block (columns per column-pass workgroup) of an ny-point transform.
+int col_block(int nx, int ny) {
+  int cb = kColElems / ny;
+  if (cb > 8) cb = 8;
+  if (cb > nx) cb = nx;
+  return cb;
+}
+
+// Workspace layout: per snapshot, one plane per spectrum, column-blocked with
+// runs of 128 bytes per row (8 complex128 or 16 complex64 columns, at least the
+// column block, at most nx). Returns the bytes per snapshot.
+long long plan_layout(int nx, int ny, QPlan* q, int nq) {
+  const int cb = col_block(nx, ny);
+  long long off = 0;
+  for (int i = 0; i < nq; ++i) {
+    int w = q[i].f32 ? 2 * kWsBlock : kWsBlock;
+    if (w < cb) w = cb;
+    if (w > nx) w = nx;
+    q[i].cbw = w;
+    q[i].lcbw = ilog2(w);
+    q[i].off = off;
+    off += (long long)nx * ny * (q[i].f32 ? 8 : 16);
+  }
+  return off;
 }
 
 struct TwCache {
@@ -743,11 +825,12 @@ int twiddles(osar_ctx* ctx, int n, const double2** out) {
 
 extern "C" int64_t osar_surface_workspace_bytes(const osar_surface_desc* d) {
   if (!d) return -1;
+  if (ilog2(d->nx) < 0 || ilog2(d->ny) < 0) return -1;
   QPlan q[kMaxQ];
   int nq = 0;
   if (make_plan(d, q, &nq) != OSAR_OK) return -1;
-  // packed spectra [nq][n_times][ny][nx], then the omega table [ny][nx]
-  return (int64_t)nq * d->n_times * (int64_t)d->nx * d->ny * (int64_t)sizeof(double2) +
+  // packed spectra [n_times][nq planes], then the omega table [ny][nx]
+  return (int64_t)plan_layout(d->nx, d->ny, q, nq) * d->n_times +
          (int64_t)d->nx * d->ny * (int64_t)sizeof(double);
 }
 
@@ -792,19 +875,16 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   a.ky = d->ky;
   a.coefs = d->coefs;
   a.times = d->times;
-  a.work = (double2*)ctx->work;
-  a.omega = (double*)(a.work + (size_t)a.nq * d->n_times * d->nx * d->ny);
+  a.snap_bytes = plan_layout(d->nx, d->ny, a.q, a.nq);
+  a.work = (unsigned char*)ctx->work;
+  a.omega = (double*)(a.work + (size_t)a.snap_bytes * d->n_times);
   int rc = twiddles(ctx, d->nx, &a.twx);
   if (rc) return rc;
   rc = twiddles(ctx, d->ny, &a.twy);
   if (rc) return rc;
-  int cb = kColElems / d->ny;
-  if (cb > 8) cb = 8;
-  if (cb > d->nx) cb = d->nx;
+  const int cb = col_block(d->nx, d->ny);
   a.cb = cb;
   a.lcb = ilog2(cb);
-  a.cbw = cb > kWsBlock ? cb : (kWsBlock < d->nx ? kWsBlock : d->nx);
-  a.lcbw = ilog2(a.cbw);
   hipStream_t s = (hipStream_t)stream;
   const size_t lds_rows = (padded(2 * (size_t)d->nx) + 128 + 2) * sizeof(double2);
   const size_t lds_cols = (padded((size_t)cb * d->ny) + 128) * sizeof(double2);

@@ -32,7 +32,8 @@ def mirror_grid(k):
     return bool(ok.all())
 
 
-def synth_fields(ctx, surf, kx, ky, coefs, times, out, fields=None, mirror=None):
+def synth_fields(ctx, surf, kx, ky, coefs, times, out, fields=None, mirror=None,
+                 slope_fp32=False):
     """osar_surface_synth: fields of surface `surf` (a SurfaceInit) at `times`.
 
     kx, ky, coefs: device tensors (float32 [nx], [ny], complex64 [ny][nx]).
@@ -40,6 +41,8 @@ def synth_fields(ctx, surf, kx, ky, coefs, times, out, fields=None, mirror=None)
     `times` is a sequence of floats (copied to the device) or a contiguous float64
     device tensor (used in place, no host round trip). Returns the device tensor
     of times, which must stay alive until the stream has run the launch.
+    slope_fp32: the Diffx/Diffy spectrum round-trips through a complex64
+    workspace (osar_surface_desc.slope_fp32).
     """
     import torch
     fields = fields or list(out.keys())
@@ -62,6 +65,7 @@ def synth_fields(ctx, surf, kx, ky, coefs, times, out, fields=None, mirror=None)
     if mirror is None:   # checked on the host copies of the uploaded tables
         mirror = mirror_grid(surf.kx_s) and mirror_grid(surf.ky_s)
     d.mirror_grid = 1 if mirror else 0
+    d.slope_fp32 = 1 if slope_fp32 else 0
     d.depth = -1.0 if surf.depth is None else float(surf.depth)
     d.current_x, d.current_y = float(surf.current[0]), float(surf.current[1])
     d.kx_res, d.ky_res = float(surf.kx_res), float(surf.ky_res)
@@ -175,6 +179,10 @@ class DeviceScene:
         self.resid_max = self._resid_bound()
         self.sr_monotonic = bool(np.all(np.diff(sc.sr) >= 0))
         self.shape = shape if shape is not None else self.default_shape(lds_budget)
+        # Bragg-only scenes read the slopes only through the Bragg amplitudes: their
+        # spectrum takes the complex64 workspace (the specular stationary-phase test
+        # keeps every field in complex128)
+        self.slope_fp32 = bool(sc.bragg_enable and not sc.spec_enable)
 
     # ------------------------------------------------------------------
     def _extend(self, rs):
@@ -386,10 +394,10 @@ class DeviceScene:
             f += ['Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
         return [n for n in _lib.FIELD_ORDER if n in f]
 
-    def synth(self, times, out, fields=None):
+    def synth(self, times, out, fields=None, slope_fp32=False):
         """Surface fields at `times` into out[name] = float32 [n_times][nx][ny] (x-major)."""
         self._keep = synth_fields(self.ctx, self.sc.surface, self.kx, self.ky, self.coefs,
-                                  times, out, fields)
+                                  times, out, fields, slope_fp32=slope_fp32)
 
     # ------------------------------------------------------------------
     def run(self, p0=0, p1=None, chunk_snaps=32, raw=None, shape=None, timer=None,
@@ -491,9 +499,9 @@ class DeviceScene:
             for n in st.names:
                 buf[n][0].copy_(src[n][st.snap_slot])
             self.synth(self.snap_times[m0 + 1:m_last + 1], {n: buf[n][1:] for n in st.names},
-                       st.names)
+                       st.names, self.slope_fp32)
         else:
-            self.synth(self.snap_times[m0:m_last + 1], buf, st.names)
+            self.synth(self.snap_times[m0:m_last + 1], buf, st.names, self.slope_fp32)
         st.snap_held, st.snap_slot = m_last, m_last - m0
         self._mark(timer, 'synth', ev)
         r0 = r1 = 0
@@ -512,7 +520,8 @@ class DeviceScene:
                 pr = int(sc.rcs_refresh[r0])
                 tmp = {n: torch.empty((1, sc.Nx, sc.Ny), dtype=torch.float32, device=self.dev)
                        for n in ('Diffx', 'Diffy')}
-                self.synth(self.snap_times[pr // K:pr // K + 1], tmp, ['Diffx', 'Diffy'])
+                self.synth(self.snap_times[pr // K:pr // K + 1], tmp, ['Diffx', 'Diffy'],
+                           self.slope_fp32)
                 self.bragg(tmp['Diffx'][0], tmp['Diffy'][0], pr, amp[0], st.rcs_sum[r0])
                 r_first += 1
             if r_first <= r1:

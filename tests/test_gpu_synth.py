@@ -55,6 +55,37 @@ def _synth_compare(sc, times, names, compute):
     return worst
 
 
+def test_synth_slopes_in_complex64_workspace(need_gpu, tmp_path_factory):
+    """slope_fp32 (the Bragg-only run loop): the displacements stay FP64 end to
+    end (bit-equal to the oracle's rounding), the slopes carry the float32
+    rounding of the row-pass result only (max error well below 1e-6 of their RMS)."""
+    import torch
+    import oracle.surface as osurf
+    from oceansar_amd.engine import DeviceScene
+    sc = _scene(tmp_path_factory, 'sf32', L=512., dx=1.)
+    ds = DeviceScene(sc)
+    assert ds.slope_fp32
+    names = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy']
+    times = [0.0, 3.3, 17.25]
+    out = {n: torch.empty((len(times), sc.Nx, sc.Ny), dtype=torch.float32, device='cuda')
+           for n in names}
+    ds.synth(times, out, names, slope_fp32=True)
+    torch.cuda.synchronize()
+    for ti, t in enumerate(times):
+        ref = osurf.synth(sc.surface, t, ('D', 'Diff'))
+        for n in names:
+            got = out[n][ti].cpu().numpy().T
+            r = ref[n]
+            same = np.mean(got == r)
+            err = np.max(np.abs(got - r)) / np.sqrt(np.mean(r.astype(np.float64) ** 2))
+            print('%s t=%g: %.4f bit-equal, max err %.2e RMS' % (n, t, same, err))
+            if n.startswith('D') and not n.startswith('Diff'):
+                assert same > 0.999, (n, t, same)
+                assert err < 1e-6, (n, t, err)
+            else:
+                assert err < 5e-7, (n, t, err)
+
+
 def test_synth_matches_oracle_square(need_gpu, tmp_path_factory):
     sc = _scene(tmp_path_factory, 's1')
     names = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
@@ -124,7 +155,10 @@ def test_end_to_end_matches_oracle(need_gpu, tmp_path_factory):
     e, ph = rel_rms(got, ref), phase_rms(got, ref)
     print('end-to-end rel RMS %.3e phase RMS %.3e' % (e, ph))
     assert e < 1e-4 and ph < 1e-3
-    assert np.allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-9)
+    # Bragg-only runs take the slopes through a complex64 workspace: the
+    # RCS sums carry its float32 rounding (the oracle's slopes are FP64)
+    print('NRCS max rel diff %.2e' % np.max(np.abs(nrcs[:, p0:p1] / ref_nrcs - 1)))
+    np.testing.assert_allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-6)
 
 
 def test_end_to_end_dp_multichannel_xti(need_gpu, tmp_path_factory):
@@ -144,7 +178,10 @@ def test_end_to_end_dp_multichannel_xti(need_gpu, tmp_path_factory):
     e, ph = rel_rms(got, ref), phase_rms(got, ref)
     print('DP/XTI rel RMS %.3e phase RMS %.3e' % (e, ph))
     assert e < 1e-4 and ph < 1e-3
-    assert np.allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-9)
+    # Bragg-only runs take the slopes through a complex64 workspace: the
+    # RCS sums carry its float32 rounding (the oracle's slopes are FP64)
+    print('NRCS max rel diff %.2e' % np.max(np.abs(nrcs[:, p0:p1] / ref_nrcs - 1)))
+    np.testing.assert_allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-6)
 
 
 @pytest.mark.parametrize('pol', ['vv', 'DP'])

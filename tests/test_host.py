@@ -169,3 +169,28 @@ def test_frozen_ocean_holds_the_surface_at_t0(tmp_path):
     assert not live.frozen and live.snapshot_time(7) == 28 * live.t_step
     r0, _ = osar.simulate(sc, 40, 44, snaps=cache)
     assert np.all(np.isfinite(r0)) and np.abs(r0).max() > 0
+
+
+def test_workspace_bytes_match_the_library(tmp_path):
+    """batch.workspace_bytes (admission and launch-group sizing) agrees with
+    osar_surface_workspace_bytes for the fields each kind of scene synthesises."""
+    import ctypes
+    from oceansar_amd import _lib, batch, configs
+    from oceansar_amd.engine import DeviceScene
+    lib = _lib.load()
+    for kw in (dict(), dict(specular=True, base='ati_swell', L=64.)):
+        base = kw.pop('base', 'small')
+        sc = configs.load_scene(configs.workload(base, L=kw.pop('L', 128.), **kw), tmp_path)
+        names = DeviceScene.fields_needed(types_ns(sc))
+        for chunk in (1, 8, 32):
+            d = _lib.SurfaceDesc()
+            d.nx, d.ny, d.n_times = sc.Nx, sc.Ny, chunk + 1
+            d.fields = sum(_lib.FIELD_BITS[n] for n in names)
+            d.slope_fp32 = 1 if (sc.bragg_enable and not sc.spec_enable) else 0
+            assert lib.osar_surface_workspace_bytes(ctypes.byref(d)) == \
+                batch.workspace_bytes(sc, chunk), (sc.spec_enable, chunk)
+
+
+def types_ns(sc):
+    import types
+    return types.SimpleNamespace(sc=sc)
