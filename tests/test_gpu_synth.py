@@ -155,8 +155,8 @@ def test_end_to_end_matches_oracle(need_gpu, tmp_path_factory):
     e, ph = rel_rms(got, ref), phase_rms(got, ref)
     print('end-to-end rel RMS %.3e phase RMS %.3e' % (e, ph))
     assert e < 1e-4 and ph < 1e-3
-    # Bragg-only runs take the slopes through a complex64 workspace: the
-    # RCS sums carry its float32 rounding (the oracle's slopes are FP64)
+    # the Bragg facet model runs in float32 and Bragg-only runs take the slopes
+    # through a complex64 workspace: the RCS sums agree to ~1e-7
     print('NRCS max rel diff %.2e' % np.max(np.abs(nrcs[:, p0:p1] / ref_nrcs - 1)))
     np.testing.assert_allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-6)
 
@@ -178,8 +178,8 @@ def test_end_to_end_dp_multichannel_xti(need_gpu, tmp_path_factory):
     e, ph = rel_rms(got, ref), phase_rms(got, ref)
     print('DP/XTI rel RMS %.3e phase RMS %.3e' % (e, ph))
     assert e < 1e-4 and ph < 1e-3
-    # Bragg-only runs take the slopes through a complex64 workspace: the
-    # RCS sums carry its float32 rounding (the oracle's slopes are FP64)
+    # the Bragg facet model runs in float32 and Bragg-only runs take the slopes
+    # through a complex64 workspace: the RCS sums agree to ~1e-7
     print('NRCS max rel diff %.2e' % np.max(np.abs(nrcs[:, p0:p1] / ref_nrcs - 1)))
     np.testing.assert_allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-6)
 
@@ -202,7 +202,9 @@ def test_end_to_end_specular(need_gpu, tmp_path_factory, pol):
     e, ph = rel_rms(got, ref), phase_rms(got, ref)
     print('specular %s rel RMS %.3e phase RMS %.3e' % (pol, e, ph))
     assert e < 1e-4 and ph < 1e-3
-    np.testing.assert_allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-9)
+    # the Bragg facet model runs in float32: its RCS sums agree to ~1e-7
+    print('NRCS max rel diff %.2e' % np.max(np.abs(nrcs[:, p0:p1] / ref_nrcs - 1)))
+    np.testing.assert_allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-6)
 
 
 def test_overlapped_pipeline_matches_serial(need_gpu, tmp_path_factory):

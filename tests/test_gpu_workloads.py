@@ -103,7 +103,9 @@ def test_ati_swell_window_matches_oracle(need_gpu, ati_scene):
     assert e < 1e-4 and ph < 1e-3
     for ch in range(sc.num_ch):
         assert rel_rms(got[:, ch], ref[:, ch]) < 1e-4, ch
-    np.testing.assert_allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-9)
+    # the Bragg facet model runs in float32: its RCS sums agree to ~1e-7
+    print('NRCS max rel diff %.2e' % np.max(np.abs(nrcs[:, p0:p1] / ref_nrcs - 1)))
+    np.testing.assert_allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-6)
 
 
 def _mix(tmp_path_factory, kws, tag):
