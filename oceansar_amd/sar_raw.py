@@ -70,14 +70,34 @@ def sar_raw(cfg_file, output_file, ocean_file, reuse_ocean_file, errors_file,
     raw = raw_dev[..., :sc.rg_samp_orig].cpu().numpy().astype(np.complex128)
     timings['download'] = time.perf_counter() - t3
 
+    raw = calibrate_and_add_noise(sc, raw)
+    t4 = time.perf_counter()
+    write_raw_file(sc, output_file, raw, nrcs)
+    timings['write'] = time.perf_counter() - t4
+    if verbose:
+        w = sc.contributions
+        print('sar_raw: %dx%d facets, %d pulses, %d ch, %s: pulse loop %.3f s '
+              '(%.3e contributions/s)' % (sc.Nx, sc.Ny, sc.az_steps, sc.num_ch, sc.pol,
+                                         timings['pulse_loop'], w / timings['pulse_loop']))
+    return timings
+
+
+def calibrate_and_add_noise(sc, raw):
+    """cal_factor and the complex noise of sar_raw_nompi.py:558-574 on the
+    trimmed raw cube [pol][ch][az][rg_orig] (complex128); the noise comes from
+    the scene's seeded noise stream."""
     raw = raw * sc.cal_factor
     namp = db2lin(sc.nesz, amplitude=True) / np.sqrt(2.)
     rng = sc.rng_noise
     for ip in range(raw.shape[0]):
         raw[ip] = raw[ip] + namp * (rng.standard_normal(raw[ip].shape) +
                                     1j * rng.standard_normal(raw[ip].shape))
+    return raw
 
-    t4 = time.perf_counter()
+
+def write_raw_file(sc, output_file, raw, nrcs):
+    """The RawFile of sar_raw_nompi.py:624-639: raw_data [pol][ch][az][rg_orig],
+    NRCS_avg [pol][az] and the scene attributes sar_focus reads back."""
     with raw_io.RawFile(output_file, 'w', raw.shape) as rf:
         rf.set('inc_angle', np.rad2deg(sc.inc_angle))
         rf.set('f0', sc.f0)
@@ -93,13 +113,6 @@ def sar_raw(cfg_file, output_file, ocean_file, reuse_ocean_file, errors_file,
         rf.set('NRCS_avg', nrcs)
         rf.set('b_ati', sc.b_ati)
         rf.set('b_xti', sc.b_xti)
-    timings['write'] = time.perf_counter() - t4
-    if verbose:
-        w = sc.contributions
-        print('sar_raw: %dx%d facets, %d pulses, %d ch, %s: pulse loop %.3f s '
-              '(%.3e contributions/s)' % (sc.Nx, sc.Ny, sc.az_steps, sc.num_ch, sc.pol,
-                                         timings['pulse_loop'], w / timings['pulse_loop']))
-    return timings
 
 
 def sarsim(cfg_file, device=0):
