@@ -242,6 +242,19 @@ typedef struct osar_surface_desc {
 int64_t osar_surface_workspace_bytes(const osar_surface_desc* d);
 int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream);
 
+/* osar_surface_synth without any device allocation. No counterpart in the
+   reference: for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+   same kernels and results as osar_surface_synth, with the synthesis workspace
+   taken from a scratch array of OSAR_SYNTH_SCRATCH_BYTES that the library holds
+   on every device it runs on until the process ends, instead of the context's
+   workspace. A descriptor whose workspace (osar_surface_workspace_bytes) exceeds
+   it is rejected with OSAR_EINVAL (split its snapshots over several calls). All
+   calls on one device share the scratch: calls that overlap on one device race
+   (serialise them on one stream). */
+#define OSAR_SYNTH_SCRATCH_BYTES (256ll << 20)
+int osar_surface_synth_noalloc(osar_ctx* ctx, const osar_surface_desc* d, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

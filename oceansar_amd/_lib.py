@@ -68,7 +68,8 @@ FIELD_ORDER = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
 EXPORTS = ['osar_ctx_create', 'osar_ctx_destroy', 'osar_last_error', 'osar_version',
            'osar_struct_size', 'osar_echo_accumulate', 'osar_fill_zero_c32',
            'osar_echo_deposit', 'osar_facet_bragg', 'osar_fill_zero_f64',
-           'osar_surface_workspace_bytes', 'osar_surface_synth']
+           'osar_surface_workspace_bytes', 'osar_surface_synth', 'osar_surface_synth_noalloc']
+SYNTH_SCRATCH_BYTES = 256 << 20    # OSAR_SYNTH_SCRATCH_BYTES
 
 _lib = None
 
@@ -97,6 +98,7 @@ def load():
     lib.osar_surface_workspace_bytes.argtypes = [POINTER(SurfaceDesc)]
     lib.osar_surface_workspace_bytes.restype = c_int64
     lib.osar_surface_synth.argtypes = [c_void_p, POINTER(SurfaceDesc), c_void_p]
+    lib.osar_surface_synth_noalloc.argtypes = [c_void_p, POINTER(SurfaceDesc), c_void_p]
     for name, cls in ((0, EchoDesc), (1, BraggDesc), (2, SurfaceDesc)):
         n = lib.osar_struct_size(name)
         if n != ctypes.sizeof(cls):

@@ -22,6 +22,10 @@ struct osar_ctx {
 // first eager run) make no other runtime calls.
 int osar_reserve_lds(const void* kernel, size_t bytes);
 
+// Per-device set-up of the synthesis (twiddle tables, no-allocation scratch),
+// once per device with that device current; called by osar_ctx_create.
+int osar_synth_device_setup(int device);
+
 // Thread-local last error message (osar_last_error).
 void osar_set_error(const char* fmt, ...);
 

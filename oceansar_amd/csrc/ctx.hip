@@ -51,6 +51,10 @@ extern "C" int osar_ctx_create(int device, osar_ctx** out) {
   OSAR_REQUIRE(device >= 0 && device < n, OSAR_EINVAL, "device %d out of range (%d devices)",
                device, n);
   OSAR_HIP(hipSetDevice(device));
+  {
+    const int rc = osar_synth_device_setup(device);
+    if (rc) return rc;
+  }
   hipDeviceProp_t prop;
   OSAR_HIP(hipGetDeviceProperties(&prop, device));
   osar_ctx* c = (osar_ctx*)calloc(1, sizeof(osar_ctx));

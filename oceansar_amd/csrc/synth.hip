@@ -25,6 +25,7 @@
 // one workgroup's HBM traffic overlaps the other's FFT.
 #include <math.h>
 
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -790,38 +791,65 @@ long long plan_layout(int nx, int ny, QPlan* q, int nq) {
   return off;
 }
 
-struct TwCache {
-  int n = 0;
-  double2* dev = nullptr;
-};
+// Two-level twiddle tables of exp(+2 pi i k / N) for every supported length N
+// = 16 .. 2048 (entries 0..63 hold k = 0..63, entries 64..127 hold k = 64 m),
+// computed on the host in long double and copied to each device once, when
+// the first context of that device is created (osar_synth_device_setup).
+constexpr int kTwLens = 8;   // log2 N = 4 .. 11
+__device__ double2 g_twiddles[kTwLens][128];
 
-// Two-level twiddle tables of exp(+2 pi i k / N), computed on the host in long
-// double: entries 0..63 hold k = 0..63, entries 64..127 hold k = 64 m.
+// Scratch of osar_surface_synth_noalloc: its synthesis workspace, one per
+// device for the life of the process (every call on a device shares it).
+static __device__ __align__(256) unsigned char scratch[OSAR_SYNTH_SCRATCH_BYTES];
+
+struct DeviceTables {
+  bool ready = false;
+  double2* twiddles = nullptr;     // device address of g_twiddles
+  unsigned char* scratch = nullptr;  // device address of scratch
+};
+constexpr int kMaxDevices = 64;
+DeviceTables g_dev[kMaxDevices];
+std::mutex g_dev_mu;
+
 int twiddles(osar_ctx* ctx, int n, const double2** out) {
-  static thread_local TwCache cache[64][16];
   const int l = ilog2(n);
-  OSAR_REQUIRE(ctx->device >= 0 && ctx->device < 64 && l >= 0 && l < 16, OSAR_EUNSUPPORTED,
-               "twiddle table for device %d, n %d", ctx->device, n);
-  TwCache& c = cache[ctx->device][l];
-  if (c.n != n || c.dev == nullptr) {
-    std::vector<double2> h(128);
-    const long double pi = 3.14159265358979323846264338327950288L;
-    for (int e = 0; e < 128; ++e) {
-      const long k = e < 64 ? e : 64L * (e - 64);
-      const long double ang = 2.0L * pi * (long double)(k % n) / (long double)n;
-      h[e] = make_double2((double)cosl(ang), (double)sinl(ang));
-    }
-    double2* p = nullptr;
-    OSAR_HIP(hipMalloc(&p, sizeof(double2) * 128));
-    OSAR_HIP(hipMemcpy(p, h.data(), sizeof(double2) * 128, hipMemcpyHostToDevice));
-    c.n = n;
-    c.dev = p;
-  }
-  *out = c.dev;
+  OSAR_REQUIRE(ctx->device >= 0 && ctx->device < kMaxDevices && l >= 4 && l < 4 + kTwLens &&
+                   g_dev[ctx->device].ready,
+               OSAR_EUNSUPPORTED, "twiddle table for device %d, n %d", ctx->device, n);
+  *out = g_dev[ctx->device].twiddles + (size_t)(l - 4) * 128;
   return OSAR_OK;
 }
 
 }  // namespace
+
+// Once per device, with that device current: the device addresses of the
+// twiddle tables and of the no-allocation scratch, and the twiddle values.
+int osar_synth_device_setup(int device) {
+  OSAR_REQUIRE(device >= 0 && device < kMaxDevices, OSAR_EUNSUPPORTED,
+               "device %d beyond the %d the library tracks", device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_dev_mu);
+  DeviceTables& t = g_dev[device];
+  if (t.ready) return OSAR_OK;
+  void* tw = nullptr;
+  void* sc = nullptr;
+  OSAR_HIP(hipGetSymbolAddress(&tw, HIP_SYMBOL(g_twiddles)));
+  OSAR_HIP(hipGetSymbolAddress(&sc, HIP_SYMBOL(scratch)));
+  std::vector<double2> h((size_t)kTwLens * 128);
+  const long double pi = 3.14159265358979323846264338327950288L;
+  for (int l = 4; l < 4 + kTwLens; ++l) {
+    const long n = 1L << l;
+    for (int e = 0; e < 128; ++e) {
+      const long k = e < 64 ? e : 64L * (e - 64);
+      const long double ang = 2.0L * pi * (long double)(k % n) / (long double)n;
+      h[(size_t)(l - 4) * 128 + e] = make_double2((double)cosl(ang), (double)sinl(ang));
+    }
+  }
+  OSAR_HIP(hipMemcpy(tw, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice));
+  t.twiddles = (double2*)tw;
+  t.scratch = (unsigned char*)sc;
+  t.ready = true;
+  return OSAR_OK;
+}
 
 extern "C" int64_t osar_surface_workspace_bytes(const osar_surface_desc* d) {
   if (!d) return -1;
@@ -834,7 +862,13 @@ extern "C" int64_t osar_surface_workspace_bytes(const osar_surface_desc* d) {
          (int64_t)d->nx * d->ny * (int64_t)sizeof(double);
 }
 
-extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream) {
+namespace {
+
+// The synthesis of descriptor d with the workspace `work` (at least
+// osar_surface_workspace_bytes(d) bytes), or with the context's workspace,
+// grown on demand, when work is NULL.
+int surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream, unsigned char* work,
+                  size_t work_bytes) {
   OSAR_PTR(ctx);
   OSAR_PTR(d);
   const int lx = ilog2(d->nx), ly = ilog2(d->ny);
@@ -856,12 +890,20 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   }
   OSAR_HIP(hipSetDevice(ctx->device));
   const size_t need = (size_t)osar_surface_workspace_bytes(d);
-  if (need > ctx->work_bytes) {
-    if (ctx->work) OSAR_HIP(hipFree(ctx->work));
-    ctx->work = nullptr;
-    ctx->work_bytes = 0;
-    OSAR_HIP(hipMalloc(&ctx->work, need));
-    ctx->work_bytes = need;
+  if (work != nullptr) {
+    OSAR_REQUIRE(need <= work_bytes, OSAR_EINVAL,
+                 "surface %dx%d with %d snapshots needs %zu B of workspace; the "
+                 "no-allocation scratch holds %zu B (split the snapshots over calls)",
+                 d->nx, d->ny, d->n_times, need, work_bytes);
+  } else {
+    if (need > ctx->work_bytes) {
+      if (ctx->work) OSAR_HIP(hipFree(ctx->work));
+      ctx->work = nullptr;
+      ctx->work_bytes = 0;
+      OSAR_HIP(hipMalloc(&ctx->work, need));
+      ctx->work_bytes = need;
+    }
+    work = (unsigned char*)ctx->work;
   }
   a.nx = d->nx;
   a.ny = d->ny;
@@ -876,7 +918,7 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
   a.coefs = d->coefs;
   a.times = d->times;
   a.snap_bytes = plan_layout(d->nx, d->ny, a.q, a.nq);
-  a.work = (unsigned char*)ctx->work;
+  a.work = work;
   a.omega = (double*)(a.work + (size_t)a.snap_bytes * d->n_times);
   int rc = twiddles(ctx, d->nx, &a.twx);
   if (rc) return rc;
@@ -930,4 +972,18 @@ extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, voi
     hipLaunchKernelGGL(synth_cols_kernel<false>, grid_cols, dim3(kColThreads), lds_cols, s, a);
   OSAR_LAUNCHED("synth_cols_kernel");
   return OSAR_OK;
+}
+
+}  // namespace
+
+extern "C" int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream) {
+  return surface_synth(ctx, d, stream, nullptr, 0);
+}
+
+extern "C" int osar_surface_synth_noalloc(osar_ctx* ctx, const osar_surface_desc* d,
+                                          void* stream) {
+  OSAR_PTR(ctx);
+  OSAR_REQUIRE(ctx->device >= 0 && ctx->device < kMaxDevices && g_dev[ctx->device].ready,
+               OSAR_EINVAL, "context of device %d has no scratch", ctx->device);
+  return surface_synth(ctx, d, stream, g_dev[ctx->device].scratch, OSAR_SYNTH_SCRATCH_BYTES);
 }
