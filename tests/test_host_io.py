@@ -45,3 +45,89 @@ def test_ocean_state_round_trip(tmp_path):
     np.testing.assert_array_equal(s2.state.y, sc.surface.y)
     assert s2.state.depth == sc.surface.depth
     assert np.isclose(s2.state.wind_U_eff, sc.surface.wind_U_eff, rtol=1e-6)
+
+
+def test_raw_file_reads_as_sar_focus_reads_it(tmp_path):
+    """The file sar_raw writes, read the way sar_focus reads it
+    (sar_processor.py:71-77: RawFile(raw_file, 'r') then get('raw_data*'),
+    get('sr0'), get('inc_angle'), get('b_ati'), get('b_xti')), with the
+    reference's NETCDFHandler.get semantics (netcdf.py:20-31: 'x*' joins x_r
+    and x_i into complex64, other names return variables[name][:]). The reader
+    here is scipy's NetCDF reader, which opens the same 64-bit-offset file
+    netCDF4.Dataset does; the values are the pulse loop's calibrated raw data."""
+    from oceansar_amd import configs
+    from oceansar_amd import sar_raw as sr
+    from oceansar_amd.utils import optimize_fftsize
+    import oracle.sar as osar
+    sc = configs.load_scene(configs.workload('small', L=32., n_pulses=48, K=4, num_ch=2,
+                                             b_ati=[0., 10.], pol='DP'), tmp_path)
+    raw, nrcs = osar.simulate(sc, 0, sc.az_steps)
+    raw = sr.calibrate_and_add_noise(sc, osar.post_process(sc, raw))
+    p = str(tmp_path / 'raw_data.nc')
+    sr.write_raw_file(sc, p, raw, nrcs)
+
+    f = netcdf_file(p, 'r', mmap=False)
+
+    def get(name):                                   # netcdf.py:20-31
+        if name[-1] == '*':
+            content = np.empty(f.variables[name[:-1] + '_r'].shape, dtype=np.complex64)
+            content.real = f.variables[name[:-1] + '_r'][:]
+            content.imag = f.variables[name[:-1] + '_i'][:]
+        else:
+            # netCDF4's [:] on a scalar variable returns its value; scipy's is getValue()
+            v = f.variables[name]
+            content = v[:] if v.shape else v.getValue()
+        return content
+
+    raw_data = get('raw_data*')
+    sr0 = get('sr0')
+    inc_angle = get('inc_angle')
+    b_ati = get('b_ati')
+    b_xti = get('b_xti')
+    nrcs_file = get('NRCS_avg')
+    assert raw_data.dtype == np.complex64
+    assert raw_data.shape == (2, 2, sc.az_steps, sc.rg_samp_orig)   # [pol, ch, az, rg]
+    np.testing.assert_array_equal(raw_data, raw.astype(np.complex64))
+    assert float(sr0) == sc.sr_near
+    assert float(inc_angle) == np.rad2deg(sc.inc_angle)
+    np.testing.assert_array_equal(b_ati, [0., 10.])
+    np.testing.assert_array_equal(b_xti, [0., 0.])
+    np.testing.assert_array_equal(nrcs_file, nrcs)
+    assert int(f.variables['num_ch'].getValue()) == 2
+    # the focusing's first steps run on what was read (sar_processor.py:131-140)
+    for ch in range(2):
+        az_size_orig, rg_size_orig = raw_data[0, ch].shape
+        optsize = optimize_fftsize(raw_data[0, ch].shape)
+        optsize = [raw_data.shape[0], optsize[0], optsize[1]]
+        data = np.zeros(optsize, dtype=complex)
+        data[:, :raw_data[0, ch].shape[0], :raw_data[0, ch].shape[1]] = raw_data[:, ch, :, :]
+        assert data.shape == (2, 64, optimize_fftsize(sc.rg_samp_orig))
+    f.close()
+
+
+def test_ocean_state_file_layout_matches_reference(tmp_path):
+    """OceanStateFile variables, dimensions and types of state.py:7-72 (Nx/Ny are
+    'i4' here: NetCDF-3 has no 'u4'), so the reference's reuse path finds them."""
+    from oceansar_amd import configs
+    from oceansar_amd.ocean import OceanSurface
+    sc = configs.load_scene(configs.workload('small', L=32.), tmp_path)
+    s = OceanSurface()
+    s._adopt(sc.surface, ['D'])
+    p = str(tmp_path / 'ocean.nc')
+    s.save(p)
+    with netcdf_file(p, 'r', mmap=False) as f:
+        v = f.variables
+        assert f.dimensions['y_dim'] == sc.Ny and f.dimensions['x_dim'] == sc.Nx
+        for n in ('Lx', 'Ly', 'dx', 'dy', 'wind_dir', 'wind_dir_eff', 'wind_fetch', 'wind_U',
+                  'wind_U_eff', 'current_mag', 'current_dir', 'swell_enable', 'swell_ampl',
+                  'swell_dir', 'swell_wl', 'swell_ph0', 'choppy_enable'):
+            assert v[n].shape == () and v[n].typecode() == 'f', n
+        for n in ('kx', 'ky'):
+            assert v[n].dimensions == ('y_dim', 'x_dim') and v[n].typecode() == 'f', n
+        for n in ('wave_coefs_r', 'wave_coefs_i'):
+            assert v[n].dimensions == ('y_dim', 'x_dim') and v[n].typecode() == 'd', n
+        assert v['Nx'].typecode() == 'i' and int(v['Nx'].getValue()) == sc.Nx
+        wc = np.empty(v['wave_coefs_r'].shape, dtype=np.complex64)
+        wc.real = v['wave_coefs_r'][:]
+        wc.imag = v['wave_coefs_i'][:]
+        np.testing.assert_array_equal(wc, sc.surface.wave_coefs)
