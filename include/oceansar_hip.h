@@ -19,6 +19,9 @@
  *                         oceansar/radarsim/range_profile.py:22-49,83-116)
  *   osar_echo_deposit    chan_profile_numba alone (range_profile.py:83-116):
  *                        deposit given complex facet values at given ranges
+ *   osar_sar_focus       the processing loop of sar_focus for one channel
+ *                        (oceansar/radarsim/sar_processor.py:131-225), the
+ *                        consumer of the raw data
  *
  * Conventions
  *   - Every pointer is a caller-owned device buffer; the library never frees
@@ -57,8 +60,8 @@ int osar_ctx_destroy(osar_ctx* ctx);
 const char* osar_last_error(void);
 /* Library version: major*10000 + minor*100 + patch. */
 int osar_version(void);
-/* sizeof of the descriptor structs (0 echo, 1 bragg, 2 surface), so that
-   bindings can check their layout; -1 for an unknown id. */
+/* sizeof of the descriptor structs (0 echo, 1 bragg, 2 surface, 3 focus), so
+   that bindings can check their layout; -1 for an unknown id. */
 int64_t osar_struct_size(int which);
 
 /* ---- echo accumulation ------------------------------------------------ */
@@ -254,6 +257,32 @@ int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream);
    (serialise them on one stream). */
 #define OSAR_SYNTH_SCRATCH_BYTES (256ll << 20)
 int osar_surface_synth_noalloc(osar_ctx* ctx, const osar_surface_desc* d, void* stream);
+
+/* ---- SAR focusing ----------------------------------------------------- */
+/* One channel of sar_focus (oceansar/radarsim/sar_processor.py:131-225):
+   the raw planes zero-padded to n_az x n_rg, FFT along range then azimuth,
+   times exp(4 pi i rcmc[a] fr[r] / c), inverse FFT along range, times h[a],
+   inverse FFT along azimuth, lines [az_lo, az_hi) and range samples [0, n_rg0)
+   kept. FP64 (complex128) throughout, like the reference's numpy arrays. The
+   1-D tables (fr = fftfreq(n_rg, 1 / rg_sampling), rcmc = sr0 / sqrt(1 - (fa
+   l0 / 2 v)^2) - sr0, h = exp(i ph_ac) weighting / beam) are the host's, as
+   the reference computes them. */
+typedef struct osar_focus_desc {
+  int32_t n_pol;            /* polarisation planes of the channel */
+  int32_t n_az0, n_rg0;     /* raw lines and samples per line */
+  int32_t n_az, n_rg;       /* padded sizes: powers of two in [16, 8192] */
+  int32_t az_lo, az_hi;     /* kept lines of the image, 0 <= az_lo < az_hi <= n_az0 */
+  int32_t reserved;
+  int64_t raw_pol_stride;   /* complex64 elements between polarisation planes of raw */
+  const osar_c32* raw;      /* [n_pol][n_az0][n_rg0] (planes raw_pol_stride apart) */
+  const double* fr;         /* [n_rg] range frequencies */
+  const double* rcmc;       /* [n_az] range migration in the Doppler domain */
+  const double* h;          /* [n_az] complex128 azimuth filter, interleaved re, im */
+  double* work;             /* [n_pol][n_az][n_rg] complex128 workspace (caller's) */
+  double* slc;              /* out: [n_pol][az_hi - az_lo][n_rg0] complex128 */
+} osar_focus_desc;
+
+int osar_sar_focus(osar_ctx* ctx, const osar_focus_desc* d, void* stream);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,13 @@ class SurfaceDesc(ctypes.Structure):
         [(n, c_void_p) for n in ('kx', 'ky', 'coefs', 'times')] + [('out', c_void_p * 8)]
 
 
+class FocusDesc(ctypes.Structure):
+    _fields_ = [(n, c_int32) for n in ('n_pol', 'n_az0', 'n_rg0', 'n_az', 'n_rg', 'az_lo',
+                                       'az_hi', 'reserved')] + \
+        [('raw_pol_stride', c_int64)] + \
+        [(n, c_void_p) for n in ('raw', 'fr', 'rcmc', 'h', 'work', 'slc')]
+
+
 FIELD_BITS = {'Dx': 1, 'Dy': 2, 'Dz': 4, 'Diffx': 8, 'Diffy': 16, 'Diffxx': 32,
               'Diffyy': 64, 'Diffxy': 128}
 FIELD_ORDER = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
@@ -68,7 +75,8 @@ FIELD_ORDER = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
 EXPORTS = ['osar_ctx_create', 'osar_ctx_destroy', 'osar_last_error', 'osar_version',
            'osar_struct_size', 'osar_echo_accumulate', 'osar_fill_zero_c32',
            'osar_echo_deposit', 'osar_facet_bragg', 'osar_fill_zero_f64',
-           'osar_surface_workspace_bytes', 'osar_surface_synth', 'osar_surface_synth_noalloc']
+           'osar_surface_workspace_bytes', 'osar_surface_synth', 'osar_surface_synth_noalloc',
+           'osar_sar_focus']
 SYNTH_SCRATCH_BYTES = 256 << 20    # OSAR_SYNTH_SCRATCH_BYTES
 
 _lib = None
@@ -99,7 +107,8 @@ def load():
     lib.osar_surface_workspace_bytes.restype = c_int64
     lib.osar_surface_synth.argtypes = [c_void_p, POINTER(SurfaceDesc), c_void_p]
     lib.osar_surface_synth_noalloc.argtypes = [c_void_p, POINTER(SurfaceDesc), c_void_p]
-    for name, cls in ((0, EchoDesc), (1, BraggDesc), (2, SurfaceDesc)):
+    lib.osar_sar_focus.argtypes = [c_void_p, POINTER(FocusDesc), c_void_p]
+    for name, cls in ((0, EchoDesc), (1, BraggDesc), (2, SurfaceDesc), (3, FocusDesc)):
         n = lib.osar_struct_size(name)
         if n != ctypes.sizeof(cls):
             raise ImportError('descriptor %s: C size %d != ctypes size %d'

@@ -83,6 +83,18 @@ choppy_enable={choppy}
 frozen_ocean={frozen}
 use_buoy_data=False
 
+[processing]
+az_weighting=1
+doppler_bw=1200.
+plot_format=png
+plot_tex=False
+plot_save=False
+plot_path=processing_plots/
+plot_raw=False
+plot_rcmc_dopp=False
+plot_rcmc_time=False
+plot_image_valid=False
+
 [srg]
 use_hmtf=False
 nesz=-30.

@@ -26,6 +26,10 @@ int osar_reserve_lds(const void* kernel, size_t bytes);
 // once per device with that device current; called by osar_ctx_create.
 int osar_synth_device_setup(int device);
 
+// Device address of the two-level twiddle table of an n-point transform (n a
+// power of two in 16 .. 8192; see fft_lds.h) of the context's device.
+int osar_twiddles(osar_ctx* ctx, int n, const double2** out);
+
 // Thread-local last error message (osar_last_error).
 void osar_set_error(const char* fmt, ...);
 

@@ -40,6 +40,7 @@ extern "C" int64_t osar_struct_size(int which) {
     case 0: return (int64_t)sizeof(osar_echo_desc);
     case 1: return (int64_t)sizeof(osar_bragg_desc);
     case 2: return (int64_t)sizeof(osar_surface_desc);
+    case 3: return (int64_t)sizeof(osar_focus_desc);
     default: return -1;
   }
 }

@@ -1,9 +1,9 @@
-"""NetCDF files of the raw-data path: RawFile and OceanStateFile.
+"""NetCDF files of the raw-data path: RawFile, OceanStateFile and ProcFile.
 
 Same variable names, dimensions, units and the '_r'/'_i' convention for
-complex values as oceansar/ocs_io/raw.py:160-250, state.py:7-72 and
-netcdf.py:4-45, so downstream readers (sar_processor.py:71-77) find what they
-expect. netCDF4 is not a dependency: files are written by
+complex values as oceansar/ocs_io/raw.py:160-250, state.py:7-72,
+processed.py:6-77 and netcdf.py:4-45, so downstream readers
+(sar_processor.py:71-77) find what they expect. netCDF4 is not a dependency: files are written by
 scipy.io.netcdf_file as NetCDF-3 64-bit-offset, which netCDF4.Dataset reads.
 NetCDF-3 has no 'u4', so the state file stores Nx/Ny as 'i4'.
 """
@@ -109,3 +109,32 @@ class OceanStateFile(_Handler):
             _var(f, 'ky', 'f4', ('y_dim', 'x_dim'), '[rad/m]')
             _var(f, 'wave_coefs_r', 'f8', ('y_dim', 'x_dim'), '[]')
             _var(f, 'wave_coefs_i', 'f8', ('y_dim', 'x_dim'), '[]')
+
+
+class ProcFile(_Handler):
+    """Processed SLC file (oceansar/ocs_io/processed.py:6-77): slc_r/_i f8
+    [ch, pol, az, rg], the radar scalars and the baselines."""
+
+    def __init__(self, file_name, mode, proc_dim=None):
+        self._f = netcdf_file(file_name, mode, version=2, mmap=False)
+        if mode == 'w':
+            f = self._f
+            f.description = 'OCEANSAR Processed SLC Data File'
+            f.history = 'Created ' + time.ctime(time.time())
+            f.source = 'OCEANSAR Simulator (oceansar_amd)'
+            if not proc_dim:
+                raise ValueError('Processed raw data dimensions are needed when creating a new file!')
+            if len(proc_dim) != 4:
+                raise ValueError('proc_dim must be (ch, pol, az, rg)')
+            for n, s in zip(('ch_dim', 'pol_dim', 'az_dim', 'rg_dim'), proc_dim):
+                f.createDimension(n, int(s))
+            _var(f, 'num_ch', 'i4', (), '[]')
+            dims = ('ch_dim', 'pol_dim', 'az_dim', 'rg_dim')
+            _var(f, 'slc_r', 'f8', dims, '[]')
+            _var(f, 'slc_i', 'f8', dims, '[]')
+            for n, u in (('inc_angle', '[deg]'), ('f0', '[Hz]'), ('ant_L', '[m]'),
+                         ('prf', '[Hz]'), ('v_ground', '[m/s]'), ('orbit_alt', '[m]'),
+                         ('sr0', '[m]'), ('rg_sampling', '[Hz]'), ('rg_bw', '[Hz]')):
+                _var(f, n, 'f8', (), u)
+            _var(f, 'b_ati', 'f8', ('ch_dim',), '[m]')
+            _var(f, 'b_xti', 'f8', ('ch_dim',), '[m]')

@@ -131,3 +131,31 @@ def test_ocean_state_file_layout_matches_reference(tmp_path):
         wc.real = v['wave_coefs_r'][:]
         wc.imag = v['wave_coefs_i'][:]
         np.testing.assert_array_equal(wc, sc.surface.wave_coefs)
+
+
+def test_focus_tables_and_proc_file(tmp_path):
+    """The host tables of the GPU focusing equal the pinned oracle's
+    (sar_processor.py:148-213), and the SLC file has the reference's
+    ProcFile layout (processed.py:6-77)."""
+    import oracle.focus as of
+    from oceansar_amd import raw_io
+    from oceansar_amd.sar_processor import FocusParams
+    fp = FocusParams(1800., 110e6, 5.405e9, 7600., 616400.5, 1200., 0.54, 1.528, 1.528)
+    fr, rcmc, h = fp.tables(8192, 1024)
+    fr2, fa2, rcmc2, h2 = of.tables(8192, 1024, 1800., 110e6, 5.405e9, 7600., 616400.5, 1200.,
+                                    0.54, 1.528, 1.528)
+    np.testing.assert_array_equal(fr, fr2)
+    np.testing.assert_array_equal(rcmc, rcmc2)
+    np.testing.assert_array_equal(h, h2)
+    assert fp.valid_rows(8192) == of.valid_rows(8192, 1200., 7600., 5.405e9, 616400.5, 1800.)
+    p = str(tmp_path / 'slc.nc')
+    slc = np.arange(2 * 1 * 3 * 4).reshape(2, 1, 3, 4) * (1 + 2j)
+    with raw_io.ProcFile(p, 'w', slc.shape) as pf:
+        pf.set('slc*', slc)
+        pf.set('num_ch', 2)
+        pf.set('b_ati', np.array([0., 10.]))
+    with netcdf_file(p, 'r', mmap=False) as f:
+        v = f.variables
+        assert v['slc_r'].dimensions == ('ch_dim', 'pol_dim', 'az_dim', 'rg_dim')
+        np.testing.assert_array_equal(v['slc_i'][:], slc.imag)
+        assert f.description.decode() == 'OCEANSAR Processed SLC Data File'
