@@ -138,9 +138,24 @@ typedef struct osar_echo_desc {
   /* out: accumulated sum of |E_spec|^2 per pulse [n_pulses_total], of the last
      polarisation (vv when present), as the driver's NRCS_avg does */
   double* spec_energy;
+  /* osar_echo_accumulate_fp64 only: FP64 amplitudes [n_rcs_slots][n_pol][2][nx][ny]
+     (osar_facet_bragg with fp64 = 1), complex128 scatterer series (interleaved
+     re, im, extended like scat_m / scat_p) and the complex128 raw cube
+     [n_pol][n_ch][n_pulses_total][nbins] */
+  const double* amp64;
+  const double* scat64_m;
+  const double* scat64_p;
+  double* raw64;
 } osar_echo_desc;
 
 int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void* stream);
+
+/* The same accumulation with every value FP64 (the inputs the reference keeps
+   in float32 aside: surface fields and the range-response table), into raw64:
+   the precision reference of osar_echo_accumulate, at the same call site
+   (sar_raw_nompi.py:349-535). Launch-shape fields are ignored. The FP64 lines
+   of a pulse (n_pol x n_ch x nbins complex128) must fit the LDS. */
+int osar_echo_accumulate_fp64(osar_ctx* ctx, const osar_echo_desc* d, void* stream);
 
 /* Zero-fill n complex64 values (a kernel, so it may be captured in a graph). */
 int osar_fill_zero_c32(osar_ctx* ctx, osar_c32* p, int64_t n, void* stream);
@@ -169,7 +184,9 @@ typedef struct osar_bragg_desc {
   int32_t spec_uniform;     /* 1: spec_k[i] == spec_k0 + i * spec_dk exactly (product
                                rounded first, as numpy.arange fills), so the search
                                computes grid points instead of loading them */
-  int32_t reserved;
+  int32_t fp64;             /* 0: the facet model in float32, amplitudes to amp;
+                               1: every value FP64, FP64 amplitudes to amp64 (the
+                               reference-precision path) */
   double spec_k0, spec_dk;  /* spec_k[0] and the grid step (spec_uniform) */
   double spec_inv_dk;       /* 1 / mean step of spec_k: starting guess of the search */
   const double* cos_inc;    /* [nx] cos and sin of the incidence angle */
@@ -196,6 +213,7 @@ typedef struct osar_bragg_desc {
   const int32_t* refresh_snap;   /* device [n_refresh] */
   int64_t diff_stride;      /* elements between snapshot slots of diffx/diffy */
   int64_t amp_stride;       /* elements between amplitude slots */
+  double* amp64;            /* out (fp64 = 1): as amp, float64 */
 } osar_bragg_desc;
 
 int osar_facet_bragg(osar_ctx* ctx, const osar_bragg_desc* d, void* stream);

@@ -38,20 +38,22 @@ class EchoDesc(ctypes.Structure):
             'sr_col', 'sin_inc', 'cos_inc', 'dinc_col', 'cap_rate', 'y_row', 'dx', 'dy', 'dz',
             'amp', 'rcs_slot', 'scat_m', 'scat_p', 'ind0_m', 'ind0_p', 'sinc_tab', 'raw',
             'sx', 'sy', 'sxx', 'syy', 'sxy', 'inc_col')] + \
-        [('facet_dx', c_double), ('facet_dy', c_double), ('spec_energy', c_void_p)]
+        [('facet_dx', c_double), ('facet_dy', c_double), ('spec_energy', c_void_p)] + \
+        [(n, c_void_p) for n in ('amp64', 'scat64_m', 'scat64_p', 'raw64')]
 
 
 class BraggDesc(ctypes.Structure):
     _fields_ = [(n, c_int32) for n in ('nx', 'ny', 'pol', 'spread', 'n_spec', 'pulse')] + \
         [(n, c_double) for n in ('k0', 'd', 'wind_dir', 'wind_U', 'wind_fetch', 'dx', 'dy',
                                  't_step', 't_span', 'v_ground', 'gr0')] + \
-        [(n, c_int32) for n in ('spec_uniform', 'reserved')] + \
+        [(n, c_int32) for n in ('spec_uniform', 'fp64')] + \
         [(n, c_double) for n in ('spec_k0', 'spec_dk', 'spec_inv_dk')] + \
         [(n, c_void_p) for n in ('cos_inc', 'sin_inc', 'y_row', 'spec_k', 'spec_S', 'spec_slope',
                                  'diffx', 'diffy', 'amp', 'rcs_sum')] + \
         [(n, c_int32) for n in ('n_refresh', 'snap0')] + \
         [(n, c_void_p) for n in ('refresh_pulse', 'refresh_snap')] + \
-        [(n, ctypes.c_int64) for n in ('diff_stride', 'amp_stride')]
+        [(n, ctypes.c_int64) for n in ('diff_stride', 'amp_stride')] + \
+        [('amp64', c_void_p)]
 
 
 class SurfaceDesc(ctypes.Structure):
@@ -73,7 +75,8 @@ FIELD_BITS = {'Dx': 1, 'Dy': 2, 'Dz': 4, 'Diffx': 8, 'Diffy': 16, 'Diffxx': 32,
 FIELD_ORDER = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
 
 EXPORTS = ['osar_ctx_create', 'osar_ctx_destroy', 'osar_last_error', 'osar_version',
-           'osar_struct_size', 'osar_echo_accumulate', 'osar_fill_zero_c32',
+           'osar_struct_size', 'osar_echo_accumulate', 'osar_echo_accumulate_fp64',
+           'osar_fill_zero_c32',
            'osar_echo_deposit', 'osar_facet_bragg', 'osar_fill_zero_f64',
            'osar_surface_workspace_bytes', 'osar_surface_synth', 'osar_surface_synth_noalloc',
            'osar_sar_focus']
@@ -98,6 +101,7 @@ def load():
     lib.osar_struct_size.argtypes = [c_int]
     lib.osar_struct_size.restype = c_int64
     lib.osar_echo_accumulate.argtypes = [c_void_p, POINTER(EchoDesc), c_void_p]
+    lib.osar_echo_accumulate_fp64.argtypes = [c_void_p, POINTER(EchoDesc), c_void_p]
     lib.osar_fill_zero_c32.argtypes = [c_void_p, c_void_p, c_int64, c_void_p]
     lib.osar_fill_zero_f64.argtypes = [c_void_p, c_void_p, c_int64, c_void_p]
     lib.osar_echo_deposit.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_double, c_double,

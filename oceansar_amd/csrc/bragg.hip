@@ -310,9 +310,191 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
   }
 }
 
-template <bool UNIFORM, int POL>
+// ---- FP64 facet model (osar_facet_bragg with fp64 = 1): the same
+// identities as the float32 kernel, every value FP64, amplitudes written as
+// FP64 square roots; the reference-precision path the float32 one is measured
+// against.
+struct cd {
+  double re, im;
+};
+__device__ __forceinline__ cd cadd(cd a, cd b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cd cmulc(cd a, cd b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cd cscale(cd a, double s) { return {a.re * s, a.im * s}; }
+__device__ __forceinline__ double cabs2(cd a) { return a.re * a.re + a.im * a.im; }
+
+// (cos, sin) of atan(t) in FP64; beyond |t| = 1e150 the limits of atan(+-inf).
+__device__ __forceinline__ void cos_sin_atan64(double t, double* c, double* s) {
+  const double r = rsqrt(1. + t * t);
+  const bool huge = fabs(t) >= 1e150;
+  *c = huge ? 0. : r;
+  *s = huge ? copysign(1., t) : t * r;
+}
+
+template <bool UNIFORM, int POL, int SPREAD>
+__global__ __launch_bounds__(kThreads) void facet_bragg_fp64_kernel(osar_bragg_desc d, BraggConst c,
+                                                               int cols_per_block) {
+  const size_t n = (size_t)d.nx * d.ny;
+  constexpr int npol = POL == 2 ? 2 : 1;
+  const int ri = blockIdx.y;
+  const int pulse = d.n_refresh > 0 ? d.refresh_pulse[ri] : d.pulse;
+  const size_t snap = d.n_refresh > 0 ? (size_t)(d.refresh_snap[ri] - d.snap0) : 0;
+  const float* diffx_s = d.diffx + snap * d.diff_stride;
+  const float* diffy_s = d.diffy + snap * d.diff_stride;
+  double* amp_s = d.amp64 + (size_t)ri * d.amp_stride;
+  const double az_now = (pulse * d.t_step - d.t_span / 2.) * d.v_ground;
+  double sums[2] = {0., 0.};
+  // Thread = one facet row; the workgroup walks a range of columns of 256
+  // consecutive rows (fields are x-major, idx = col * ny + row, so every
+  // column is one contiguous read per wave). The row's azimuth terms are
+  // computed once: az = asin(arg) (sar_raw_nompi.py:363) as (sin az, cos az),
+  // and A = az - wind_dir as (cos A, sin A).
+  const int nrb = (d.ny + kThreads - 1) / kThreads;
+  const int row = (int)(blockIdx.x % nrb) * kThreads + threadIdx.x;
+  const int cs = (int)(blockIdx.x / nrb);
+  const int col_lo = cs * cols_per_block;
+  const int col_hi = row < d.ny ? min(d.nx, col_lo + cols_per_block) : col_lo;
+  const double arg = row < d.ny ? (d.y_row[row] - az_now) / d.gr0 : 0.;
+  const double sin_az = fabs(arg) <= 1. ? arg : NAN;
+  const double cos_az = sqrt(1. - arg * arg);
+  const double cos_A = cos_az * c.cos_wd + sin_az * c.sin_wd;
+  const double sin_A = sin_az * c.cos_wd - cos_az * c.sin_wd;
+  __shared__ double sec_inc[kMaxColsPerBlock];   // 1 / cos(inc) of the workgroup's columns
+  for (int i = threadIdx.x; i < min(d.nx - col_lo, cols_per_block); i += kThreads)
+    sec_inc[i] = 1. / d.cos_inc[col_lo + i];
+  __syncthreads();
+  for (int col = col_lo; col < col_hi; ++col) {
+    const size_t idx = (size_t)col * d.ny + row;
+    const double cos_inc = d.cos_inc[col], sin_inc = d.sin_inc[col];
+    const double diffx = (double)diffx_s[idx];
+    const double diffy = (double)diffy_s[idx];
+    // parallel and orthogonal slopes s_p = atan(u), s_t = atan(v) (|u|, |v| are
+    // below 1e39: float32 slopes), with 1 / cos(s_p) = (1 + u^2) cos(s_p)
+    const double u = cos_az * diffx + sin_az * diffy;
+    const double r_sp = rsqrt(1. + u * u);
+    const double cos_sp = r_sp, sin_sp = u * r_sp, sec_sp = (1. + u * u) * r_sp;
+    double cos_st, sin_st;
+    cos_sin_atan64(-sin_az * diffx + cos_az * diffy, &cos_st, &sin_st);
+    const double cos_s_t_2 = cos_st * cos_st;
+    const double sin_s_t_2 = 1 - cos_s_t_2;
+    const double sin_ip = sin_inc * cos_sp - cos_inc * sin_sp;   // sin(inc - s_p)
+    const double cos_ip = cos_inc * cos_sp + sin_inc * sin_sp;   // cos(inc - s_p)
+    const double cos_ip_2 = cos_ip * cos_ip;
+    const double sin_ip_2 = sin_ip * sin_ip;
+    const double cos_ti = cos_ip * cos_st;
+    const double sin_ti_2 = 1 - cos_ti * cos_ti;
+    const cd eps = {73., 18.};
+    const cd sq = {c.sq_re, c.sq_im};
+    // b_hh = eps / d1^2 and b_vv = eps^2 (1 + sin_ti^2) / d2^2 as u conj(q) / |q|^2
+    // (q = d^2); the reciprocals are taken below, with the facet's others
+    const cd d1 = cadd({cos_ti, 0.}, sq), d2 = cadd(cscale(eps, cos_ti), sq);
+    const cd q1 = cmulc(d1, d1), q2 = cmulc(d2, d2);
+    const double n1 = cabs2(q1), n2 = cabs2(q2);
+    const cd u_hh = cmulc(eps, {q1.re, -q1.im});
+    const cd u_vv = cmulc(cscale(cmulc(eps, eps), 1. + sin_ti_2), {q2.re, -q2.im});
+    const double wgt = cos_ip * (sec_inc[col - col_lo] * sec_sp);   // cos_ip / (cos_inc cos_sp)
+    const double kc = d.k0 * cos_ti;
+    const double F0 = 8 * M_PI * (kc * kc) * (kc * kc);
+    // k_b = 2 k0 sqrt(zb) and 1 / k_b from one reciprocal square root
+    const double zb = sin_ip_2 + cos_ip_2 * sin_s_t_2;
+    const double rz = rsqrt(zb);   // +inf at zb = 0
+    double k_b = zb > 0. ? 2 * d.k0 * (zb * rz) : 0.;
+    double inv_kb = rz * c.inv_2k0;
+    if (k_b > 2 * d.k0) {
+      k_b = 0.;
+      inv_kb = INFINITY;
+    }
+    // phi_b = az + atan(q), q = -cos_ip sin_st / sin_ip; th0 = phi_b - wind_dir;
+    // cos(th0) from angle sums. 1 + q^2 = zb / sin_ip^2, so cos(atan q) =
+    // |sin_ip| / sqrt(zb) and sin(atan q) = -cos_ip sin_st sign(sin_ip) / sqrt(zb)
+    const double cos_q = fabs(sin_ip) * rz;
+    const double sin_q = -cos_ip * sin_st * copysign(rz, sin_ip);
+    const double cth0 = cos_A * cos_q - sin_A * sin_q;
+    const bool bad = 4 * sin_ip_2 < d.d * d.d;
+    const double S1 = interp_fill1<UNIFORM>(k_b, d.spec_k, d.spec_S, d.spec_slope, d.n_spec,
+                                            d.spec_k0, d.spec_dk, d.spec_inv_dk);
+    double x = 0., e = 0.;
+    if (SPREAD == 0) {
+      const double y = kG * inv_kb + kSurfT / kRho * k_b;
+      const double cK = sqrt(y);
+      // (cK / cK_p)^2.5 = y sqrt(cK) cK_p^-2.5 and (cK_m / cK)^2.5 = cK_m^2.5
+      // cK^-2.5, from r = 1 / sqrt(cK); k_b = 0 gives x = +inf as before
+      const double r = rsqrt(cK);
+      const double r2 = r * r;
+      x = c.a_0 + 4. * (y * (cK * r) * c.inv_cK_p25) + c.a_m * (c.cK_m25 * (r2 * r2 * r));
+      if (!(y < INFINITY)) x = INFINITY;
+      // tanh(x) = sign(x) (1 - e) / (1 + e), e = exp(-2 |x|): one exponential,
+      // no branches, within 3 ulp of the library tanh (x >= ln(2) / 4 here)
+      e = exp(-2. * fabs(x));
+    }
+    // One division gives 1 / |q1|^2, 1 / |q2|^2, 1 / sin_ti^2 and (Elfouhaily)
+    // 1 / (1 + e): |q| is between 8^4 and 100^4, 1 + e in [1, 2], so the
+    // product neither overflows nor underflows; each reciprocal is within a few
+    // ulp of its own division
+    const double ope = SPREAD == 0 ? 1. + e : 1.;
+    const double n12 = n1 * n2;
+    const double inv = 1. / (n12 * (sin_ti_2 * ope));
+    const double inv_ti = (n12 * ope) * inv;
+    const double inv_n12 = (sin_ti_2 * ope) * inv;
+    const cd b_hh = cscale(u_hh, n2 * inv_n12), b_vv = cscale(u_vv, n1 * inv_n12);
+    const double delta = SPREAD == 0 ? copysign((1. - e) * ((n12 * sin_ti_2) * inv), x) : 0.;
+    const double a = sin_s_t_2 * inv_ti;
+    const double b = sin_ip_2 * cos_s_t_2 * inv_ti;
+    const double T_vv = wgt * (F0 * cabs2(cadd(cscale(b_vv, b), cscale(b_hh, a))));
+    const double T_hh = wgt * (F0 * cabs2(cadd(cscale(b_hh, b), cscale(b_vv, a))));
+    const double base = S1 * inv_kb;
+    for (int comp = 0; comp < 2; ++comp) {
+      const double cth = comp == 0 ? cth0 : -cth0;
+      double g;
+      constexpr double kInvPi = 0.31830988618379067154;
+      if (SPREAD == 0) g = cth > 0. ? (1. + delta * (2. * cth * cth - 1.)) * kInvPi : 0.;
+      else g = cth < 0. ? 0. : kInvPi * cth * cth;
+      const double spectrum = base * g;
+      // pol 2 (DP): slot 0 = hh, slot 1 = vv
+      for (int ip = 0; ip < npol; ++ip) {
+        const bool is_hh = (POL == 1) || (POL == 2 && ip == 0);
+        const double T = is_hh ? T_hh : T_vv;
+        double rcs = T * spectrum * c.dxdy;
+        // romeiser97.py:366-392 (DP: hh masks only the plus component)
+        const bool mask = bad && !(POL == 2 && is_hh && comp == 0);
+        if (mask) rcs = 0.;
+        amp_s[((size_t)ip * 2 + comp) * n + idx] = sqrt(rcs);
+        sums[ip] += rcs;
+      }
+    }
+  }
+  // wave, then workgroup reduction of the RCS sums
+  __shared__ double red[kThreads / 64][2];
+  for (int ip = 0; ip < 2; ++ip)
+    for (int off = 32; off > 0; off >>= 1) sums[ip] += __shfl_xor(sums[ip], off);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wave][0] = sums[0];
+    red[wave][1] = sums[1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && d.rcs_sum) {
+    for (int ip = 0; ip < npol; ++ip) {
+      double v = 0.;
+      for (int w = 0; w < kThreads / 64; ++w) v += red[w][ip];
+      atomicAdd(&d.rcs_sum[(size_t)ri * npol + ip], v);
+    }
+  }
+}
+
+template <bool FP64, bool UNIFORM, int POL>
 void launch_bragg_pol(int spread, dim3 grid, hipStream_t s, const osar_bragg_desc& d,
                       const BraggConst& c, int cpb) {
+  if (FP64) {
+    if (spread == 0)
+      hipLaunchKernelGGL((facet_bragg_fp64_kernel<UNIFORM, POL, 0>), grid, dim3(kThreads), 0, s,
+                         d, c, cpb);
+    else
+      hipLaunchKernelGGL((facet_bragg_fp64_kernel<UNIFORM, POL, 1>), grid, dim3(kThreads), 0, s,
+                         d, c, cpb);
+    return;
+  }
   if (spread == 0)
     hipLaunchKernelGGL((facet_bragg_kernel<UNIFORM, POL, 0>), grid, dim3(kThreads), 0, s, d, c,
                        cpb);
@@ -321,18 +503,23 @@ void launch_bragg_pol(int spread, dim3 grid, hipStream_t s, const osar_bragg_des
                        cpb);
 }
 
-template <bool UNIFORM>
+template <bool FP64, bool UNIFORM>
 void launch_bragg_u(int pol, int spread, dim3 grid, hipStream_t s, const osar_bragg_desc& d,
                     const BraggConst& c, int cpb) {
-  if (pol == 0) launch_bragg_pol<UNIFORM, 0>(spread, grid, s, d, c, cpb);
-  else if (pol == 1) launch_bragg_pol<UNIFORM, 1>(spread, grid, s, d, c, cpb);
-  else launch_bragg_pol<UNIFORM, 2>(spread, grid, s, d, c, cpb);
+  if (pol == 0) launch_bragg_pol<FP64, UNIFORM, 0>(spread, grid, s, d, c, cpb);
+  else if (pol == 1) launch_bragg_pol<FP64, UNIFORM, 1>(spread, grid, s, d, c, cpb);
+  else launch_bragg_pol<FP64, UNIFORM, 2>(spread, grid, s, d, c, cpb);
 }
 
-void launch_bragg(bool uniform, int pol, int spread, dim3 grid, hipStream_t s,
+void launch_bragg(bool fp64, bool uniform, int pol, int spread, dim3 grid, hipStream_t s,
                   const osar_bragg_desc& d, const BraggConst& c, int cpb) {
-  if (uniform) launch_bragg_u<true>(pol, spread, grid, s, d, c, cpb);
-  else launch_bragg_u<false>(pol, spread, grid, s, d, c, cpb);
+  if (fp64) {
+    if (uniform) launch_bragg_u<true, true>(pol, spread, grid, s, d, c, cpb);
+    else launch_bragg_u<true, false>(pol, spread, grid, s, d, c, cpb);
+  } else {
+    if (uniform) launch_bragg_u<false, true>(pol, spread, grid, s, d, c, cpb);
+    else launch_bragg_u<false, false>(pol, spread, grid, s, d, c, cpb);
+  }
 }
 
 }  // namespace
@@ -352,7 +539,8 @@ extern "C" int osar_facet_bragg(osar_ctx* ctx, const osar_bragg_desc* d, void* s
   OSAR_PTR(d->spec_S);
   OSAR_PTR(d->diffx);
   OSAR_PTR(d->diffy);
-  OSAR_PTR(d->amp);
+  if (d->fp64) OSAR_PTR(d->amp64);
+  else OSAR_PTR(d->amp);
   OSAR_HIP(hipSetDevice(ctx->device));
   OSAR_REQUIRE(!d->spec_uniform || d->spec_dk > 0., OSAR_EINVAL,
                "spec_uniform needs spec_dk > 0");
@@ -383,7 +571,8 @@ extern "C" int osar_facet_bragg(osar_ctx* ctx, const osar_bragg_desc* d, void* s
   const int cpb = (int)(((size_t)d->nx + ncs - 1) / ncs);
   ncs = ((size_t)d->nx + cpb - 1) / cpb;
   const dim3 grid((unsigned)(nrb * ncs), (unsigned)n_ref);
-  launch_bragg(d->spec_uniform != 0, d->pol, d->spread, grid, (hipStream_t)stream, *d, c, cpb);
+  launch_bragg(d->fp64 != 0, d->spec_uniform != 0, d->pol, d->spread, grid, (hipStream_t)stream,
+               *d, c, cpb);
   OSAR_LAUNCHED("facet_bragg_kernel");
   return OSAR_OK;
 }

@@ -752,6 +752,143 @@ __global__ void echo_deposit_kernel(const double* srg, const osar_c32* scene, in
   }
 }
 
+
+// ---- All-FP64 reference accumulator (osar_echo_accumulate_fp64) ----------
+// The same pulse body in FP64 from end to end: FP64 Bragg amplitudes and
+// complex128 scatterer series, FP64 sin/cos of the full phases, FP64 beam,
+// channel and cross-track factors, FP64 taps deposits into FP64 LDS range
+// lines and a complex128 raw cube. Only the inputs the reference itself holds
+// in float32 stay float32: the surface fields (interpolated between snapshots
+// in float32 as the balancer does) and the range-response table. It is the
+// precision reference of the fused kernel, not a fast path: a workgroup takes
+// one pulse and kFp64Rows rows of facets, every facet deposits its 8 taps per
+// channel and polarisation with LDS atomics, and the workgroup's lines are
+// added to the cube once.
+constexpr int kFp64Threads = 256;
+constexpr int kFp64Rows = 16;
+
+template <int NPOL, bool SPEC>
+__global__ __launch_bounds__(kFp64Threads) void echo_fp64_kernel(osar_echo_desc d, int n_rowtiles) {
+  // the reference's rounding of every product and sum (phases of ~1e4 rad
+  // carry ulps of ~1e-12 rad, which a fused multiply-add would move)
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  const int nch = d.n_ch, nb = d.nbins;
+  double* lines = (double*)lds_raw;                  // [NPOL][nch][nb][2]
+  const int p = d.pulse0 + (int)(blockIdx.x / n_rowtiles);
+  const int y0 = (int)(blockIdx.x % n_rowtiles) * kFp64Rows;
+  const int rows = min(kFp64Rows, d.ny - y0);
+  for (int i = threadIdx.x; i < NPOL * nch * nb * 2; i += kFp64Threads) lines[i] = 0.;
+  __syncthreads();
+  const int m = p / d.K, rk = p - m * d.K;
+  const float wh = (float)((double)rk / (double)d.K);
+  const float wl = (float)(1.0 - (double)wh);
+  const size_t plane = (size_t)d.nx * d.ny;
+  const size_t s0 = (size_t)(m - d.snap0) * plane;
+  const size_t s1 = rk != 0 ? s0 + plane : s0;
+  const bool bragg = (d.flags & OSAR_ECHO_BRAGG) != 0;
+  const bool xti = (d.flags & OSAR_ECHO_XTI) != 0;
+  const int rslot = bragg ? d.rcs_slot[p] - d.rcs0 : 0;
+  const bool bragg_ok = bragg && rslot >= 0 && rslot < d.n_rcs_slots;
+  const double az_now = (p * d.t_step - d.t_span / 2.) * d.v_ground;
+  const double binsize = d.binsize;
+  const int spol = NPOL == 2 ? 2 : (d.pol == 1 ? 1 : 0);
+  double spec_acc = 0.;
+  const int nf = rows * d.nx;
+  for (int f = threadIdx.x; f < nf; f += kFp64Threads) {
+    const int x = f / rows, yrow = y0 + (f - x * rows);
+    const size_t idx = (size_t)x * d.ny + yrow;
+    const double az = d.y_row[yrow] - az_now;
+    const double sin_az = az / d.sr0;
+    const double mig = (az / 2) * sin_az;
+    const double Dx = lerp2(d.dx[s0 + idx], d.dx[s1 + idx], rk, wl, wh);
+    const double Dy = lerp2(d.dy[s0 + idx], d.dy[s1 + idx], rk, wl, wh);
+    const double Dz = lerp2(d.dz[s0 + idx], d.dz[s1 + idx], rk, wl, wh);
+    const Geo g = facet_range(d.sr_col[x], d.cos_inc[x], d.sin_inc[x], mig, sin_az, Dx, Dy, Dz,
+                              d.sr0);
+    const double xb = (g.srs - d.min_sr) / binsize;
+    if (!(fabs(xb) < 1.0e9)) continue;
+    const double fl = floor(xb);
+    const int dbin = (int)fl;
+    const int sb = min(max((int)rint(20.0 * (1.0 - (xb - fl))), 0), kOvs);
+    zd scene[NPOL];
+#pragma unroll
+    for (int pol = 0; pol < NPOL; ++pol) scene[pol] = {0., 0.};
+    if (bragg_ok) {
+      const double surf = -(2 * d.k0) * g.srs;
+      const double cap = d.cap_rate[x] * (double)(p + 1);
+      double s_m, c_m, s_p, c_p;
+      sincos(surf - cap, &s_m, &c_m);
+      sincos(surf + cap, &s_p, &c_p);
+      const int im = d.ind0_m[idx] + p, ip = d.ind0_p[idx] + p;
+      const zd sm = {d.scat64_m[2 * (size_t)im], d.scat64_m[2 * (size_t)im + 1]};
+      const zd sp = {d.scat64_p[2 * (size_t)ip], d.scat64_p[2 * (size_t)ip + 1]};
+      const zd b0 = zmul({c_m, s_m}, sm), b1 = zmul({c_p, s_p}, sp);
+#pragma unroll
+      for (int pol = 0; pol < NPOL; ++pol) {
+        const double* a = d.amp64 + ((size_t)(rslot * NPOL + pol) * 2) * plane + idx;
+        scene[pol] = zadd(zscl(b0, a[0]), zscl(b1, a[plane]));
+      }
+    }
+    if (SPEC) {
+      const double phi = asin(az / d.gr0);
+      zd e[2];
+      if (ka_spa(d.k0, spol, d.inc_col[x], phi, lerp2(d.sx[s0 + idx], d.sx[s1 + idx], rk, wl, wh),
+                 lerp2(d.sy[s0 + idx], d.sy[s1 + idx], rk, wl, wh),
+                 lerp2(d.sxx[s0 + idx], d.sxx[s1 + idx], rk, wl, wh),
+                 lerp2(d.syy[s0 + idx], d.syy[s1 + idx], rk, wl, wh),
+                 lerp2(d.sxy[s0 + idx], d.sxy[s1 + idx], rk, wl, wh), d.facet_dx, d.facet_dy,
+                 e)) {
+        double ps, pcs;
+        sincos(-(2. * d.k0 * g.srs), &ps, &pcs);
+        const double c15 = 44.546623974653535;  // (4 pi)^1.5
+#pragma unroll
+        for (int pol = 0; pol < NPOL; ++pol) {
+          const zd esn = zscl(zmul({pcs, ps}, e[pol]), c15);
+          scene[pol] = zadd(scene[pol], esn);
+          if (pol == NPOL - 1) spec_acc += esn.re * esn.re + esn.im * esn.im;
+        }
+      }
+    }
+    const double beam = np_sinc(d.ltx_over_l * sin_az) * np_sinc(d.lrx_over_l * sin_az);
+    for (int ch = 0; ch < nch; ++ch) {
+      double sn, cs;
+      sincos(-(d.k0 * d.b_ati[ch]) * sin_az, &sn, &cs);
+      zd fch = {beam * cs, beam * sn};
+      if (xti && d.b_xti[ch] != 0.0) {
+        double xs, xc;
+        sincos(-(d.k0 * d.b_xti[ch]) * (d.dinc_col[x] + g.dinc), &xs, &xc);
+        fch = zmul(fch, {xc, xs});
+      }
+#pragma unroll
+      for (int pol = 0; pol < NPOL; ++pol) {
+        const zd v = zmul(scene[pol], fch);
+        double* l = lines + (size_t)(pol * nch + ch) * nb * 2;
+        for (int j = 0; j < kTaps; ++j) {
+          const int b = dbin - kTaps / 2 + j;
+          if (b < 0 || b >= nb) continue;
+          const double t = (double)d.sinc_tab[sb + kOvs * j];
+          atomicAdd(&l[2 * b], t * v.re);
+          atomicAdd(&l[2 * b + 1], t * v.im);
+        }
+      }
+    }
+  }
+  if (SPEC) {
+    for (int off = 32; off > 0; off >>= 1) spec_acc += __shfl_xor(spec_acc, off);
+    if ((threadIdx.x & 63) == 0 && d.spec_energy) atomicAdd(&d.spec_energy[p], spec_acc);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < NPOL * nch * nb; i += kFp64Threads) {
+    const int line = i / nb, b = i - line * nb;
+    const double re = lines[2 * (size_t)i], im = lines[2 * (size_t)i + 1];
+    if (re == 0. && im == 0.) continue;
+    double* o = d.raw64 + (((size_t)line * d.n_pulses_total + p) * nb + b) * 2;
+    atomicAdd(&o[0], re);
+    atomicAdd(&o[1], im);
+  }
+}
+
 }  // namespace
 
 extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void* stream) {
@@ -851,5 +988,83 @@ extern "C" int osar_echo_deposit(osar_ctx* ctx, const double* srg, const osar_c3
                      (hipStream_t)stream, srg, scene, n, binsize, min_sr, sinc_tab,
                      n_sinc_samples, sinc_ovs, out, nbins);
   OSAR_LAUNCHED("echo_deposit_kernel");
+  return OSAR_OK;
+}
+
+extern "C" int osar_echo_accumulate_fp64(osar_ctx* ctx, const osar_echo_desc* d, void* stream) {
+  OSAR_PTR(ctx);
+  OSAR_PTR(d);
+  OSAR_REQUIRE(d->nx > 0 && d->ny > 0 && d->nbins > 0, OSAR_EINVAL, "empty scene");
+  OSAR_REQUIRE(d->n_ch >= 1 && d->n_ch <= OSAR_MAX_CH, OSAR_EINVAL, "n_ch must be 1..%d",
+               OSAR_MAX_CH);
+  OSAR_REQUIRE(d->n_pol == 1 || d->n_pol == 2, OSAR_EINVAL, "n_pol must be 1 or 2");
+  OSAR_REQUIRE(d->K >= 1, OSAR_EINVAL, "K must be >= 1");
+  OSAR_REQUIRE(d->pulse0 >= 0 && d->npulses >= 0 &&
+                   d->pulse0 + d->npulses <= d->n_pulses_total,
+               OSAR_EINVAL, "pulse window [%d, %d) outside [0, %d)", d->pulse0,
+               d->pulse0 + d->npulses, d->n_pulses_total);
+  if (d->npulses == 0) return OSAR_OK;
+  const int m_first = d->pulse0 / d->K;
+  const int p_last = d->pulse0 + d->npulses - 1;
+  const int m_last = p_last / d->K + ((p_last % d->K) ? 1 : 0);
+  OSAR_REQUIRE(m_first >= d->snap0 && m_last < d->snap0 + d->n_snap_slots, OSAR_EINVAL,
+               "snapshots %d..%d not in slots [%d, %d)", m_first, m_last, d->snap0,
+               d->snap0 + d->n_snap_slots);
+  OSAR_PTR(d->sr_col);
+  OSAR_PTR(d->sin_inc);
+  OSAR_PTR(d->cos_inc);
+  OSAR_PTR(d->y_row);
+  OSAR_PTR(d->dx);
+  OSAR_PTR(d->dy);
+  OSAR_PTR(d->dz);
+  OSAR_PTR(d->sinc_tab);
+  OSAR_PTR(d->raw64);
+  if (d->flags & OSAR_ECHO_BRAGG) {
+    OSAR_PTR(d->amp64);
+    OSAR_PTR(d->rcs_slot);
+    OSAR_PTR(d->scat64_m);
+    OSAR_PTR(d->scat64_p);
+    OSAR_PTR(d->ind0_m);
+    OSAR_PTR(d->ind0_p);
+    OSAR_PTR(d->cap_rate);
+  }
+  if (d->flags & OSAR_ECHO_XTI) OSAR_PTR(d->dinc_col);
+  if (d->flags & OSAR_ECHO_SPECULAR) {
+    OSAR_PTR(d->sx);
+    OSAR_PTR(d->sy);
+    OSAR_PTR(d->sxx);
+    OSAR_PTR(d->syy);
+    OSAR_PTR(d->sxy);
+    OSAR_PTR(d->inc_col);
+  }
+  OSAR_REQUIRE(d->pol >= 0 && d->pol <= 2 && (d->pol == 2) == (d->n_pol == 2), OSAR_EINVAL,
+               "pol %d inconsistent with n_pol %d", d->pol, d->n_pol);
+  const size_t lds = sizeof(double) * 2 * (size_t)d->n_pol * d->n_ch * d->nbins;
+  OSAR_REQUIRE(lds <= ctx->max_lds, OSAR_EUNSUPPORTED,
+               "FP64 range lines need %zu B of LDS (limit %zu)", lds, ctx->max_lds);
+  OSAR_HIP(hipSetDevice(ctx->device));
+  const int n_rowtiles = (d->ny + kFp64Rows - 1) / kFp64Rows;
+  const int64_t n_blocks = (int64_t)d->npulses * n_rowtiles;
+  OSAR_REQUIRE(n_blocks < INT_MAX, OSAR_EINVAL, "too many workgroups (%lld)",
+               (long long)n_blocks);
+  const bool spec = (d->flags & OSAR_ECHO_SPECULAR) != 0;
+  const void* k = d->n_pol == 2 ? (spec ? (const void*)echo_fp64_kernel<2, true>
+                                        : (const void*)echo_fp64_kernel<2, false>)
+                                : (spec ? (const void*)echo_fp64_kernel<1, true>
+                                        : (const void*)echo_fp64_kernel<1, false>);
+  {
+    const int rc = osar_reserve_lds(k, lds);
+    if (rc) return rc;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid((unsigned)n_blocks), block(kFp64Threads);
+  if (d->n_pol == 2) {
+    if (spec) hipLaunchKernelGGL((echo_fp64_kernel<2, true>), grid, block, lds, s, *d, n_rowtiles);
+    else hipLaunchKernelGGL((echo_fp64_kernel<2, false>), grid, block, lds, s, *d, n_rowtiles);
+  } else {
+    if (spec) hipLaunchKernelGGL((echo_fp64_kernel<1, true>), grid, block, lds, s, *d, n_rowtiles);
+    else hipLaunchKernelGGL((echo_fp64_kernel<1, false>), grid, block, lds, s, *d, n_rowtiles);
+  }
+  OSAR_LAUNCHED("echo_fp64_kernel");
   return OSAR_OK;
 }

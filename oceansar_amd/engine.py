@@ -184,6 +184,15 @@ class DeviceScene:
         # keeps every field in complex128)
         self.slope_fp32 = bool(sc.bragg_enable and not sc.spec_enable)
 
+    def fp64_tables(self):
+        """complex128 scatterer series of the all-FP64 reference path (made once)."""
+        import torch
+        if not hasattr(self, 'scat64_m'):
+            n = lambda rs: rs.s[np.arange(rs.N + self.n_pulses) % rs.N].astype(np.complex128)
+            self.scat64_m = _t(n(self.sc.scat_m), torch.complex128, self.dev)
+            self.scat64_p = _t(n(self.sc.scat_p), torch.complex128, self.dev)
+        return self.scat64_m, self.scat64_p
+
     # ------------------------------------------------------------------
     def _extend(self, rs):
         """s_ext[i] = s[i mod N] for i < N + n_pulses, so the kernel indexes ind0 + p."""
@@ -253,12 +262,17 @@ class DeviceScene:
         raise RuntimeError('range window of %d bins does not fit the LDS even at P=2' % W)
 
     # ------------------------------------------------------------------
-    def new_raw(self):
+    def new_raw(self, fp64=False):
+        """Zeroed raw cube [pol][ch][pulse][bin]: complex64, or complex128 for the
+        all-FP64 reference path."""
         import torch
         sc = self.sc
         raw = torch.empty((self.npol, sc.num_ch, self.n_pulses, sc.rg_samp),
-                          dtype=torch.complex64, device=self.dev)
-        self.zero_c32(raw)
+                          dtype=torch.complex128 if fp64 else torch.complex64, device=self.dev)
+        if fp64:
+            self.zero_f64(raw.view(torch.float64))
+        else:
+            self.zero_c32(raw)
         return raw
 
     def zero_c32(self, t):
@@ -270,27 +284,39 @@ class DeviceScene:
                                                    _lib.stream_ptr()))
 
     def bragg(self, diffx, diffy, pulse, amp_out, rcs_sum_out=None):
-        """sqrt(RCS) of every facet at refresh pulse `pulse` into amp_out [npol][2][Ny][Nx]."""
+        """sqrt(RCS) of every facet at refresh pulse `pulse` into amp_out [npol][2][Nx][Ny]
+        (float32; float64 runs the FP64 facet model)."""
+        import torch
         d = self._bragg_desc()
         d.pulse = int(pulse)
         d.diffx, d.diffy = _lib.ptr(diffx), _lib.ptr(diffy)
-        d.amp = _lib.ptr(amp_out)
+        d.fp64 = 1 if amp_out.dtype == torch.float64 else 0
+        if d.fp64:
+            d.amp64 = _lib.ptr(amp_out)
+        else:
+            d.amp = _lib.ptr(amp_out)
         d.rcs_sum = _lib.ptr(rcs_sum_out)
         _lib.check(self.ctx.lib.osar_facet_bragg(self.ctx.h, ctypes.byref(d), _lib.stream_ptr()))
 
     def bragg_refreshes(self, r_first, r_last, diffx, diffy, snap0, amp_out, rcs_sum_out):
         """Refreshes r_first..r_last in one launch: slopes from snapshot slots
         (diffx/diffy [slot][Nx][Ny], slot 0 = snapshot snap0), amplitudes into
-        amp_out[0 .. r_last - r_first], sums into rcs_sum_out[r_first .. r_last]."""
+        amp_out[0 .. r_last - r_first] (float32, or float64 for the FP64 facet
+        model), sums into rcs_sum_out[r_first .. r_last]."""
+        import torch
         n = r_last - r_first + 1
         d = self._bragg_desc()
+        d.fp64 = 1 if amp_out.dtype == torch.float64 else 0
         d.n_refresh = n
         d.snap0 = int(snap0)
         d.refresh_pulse = self.refresh_pulse.data_ptr() + 4 * r_first
         d.refresh_snap = self.refresh_snap.data_ptr() + 4 * r_first
         d.diffx, d.diffy = _lib.ptr(diffx), _lib.ptr(diffy)
         d.diff_stride = diffx[0].numel()
-        d.amp = _lib.ptr(amp_out)
+        if d.fp64:
+            d.amp64 = _lib.ptr(amp_out)
+        else:
+            d.amp = _lib.ptr(amp_out)
         d.amp_stride = amp_out[0].numel()
         d.rcs_sum = rcs_sum_out[r_first].data_ptr()
         _lib.check(self.ctx.lib.osar_facet_bragg(self.ctx.h, ctypes.byref(d), _lib.stream_ptr()))
@@ -379,9 +405,25 @@ class DeviceScene:
 
     def accumulate(self, dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape=None,
                    spec=None, spec_energy=None):
-        """Add pulses [pulse0, pulse0+npulses) of every channel/pol into raw (device)."""
-        d = self.echo_desc(dx, dy, dz, snap0, amp, rcs0, pulse0, npulses, raw, shape,
-                           spec, spec_energy)
+        """Add pulses [pulse0, pulse0+npulses) of every channel/pol into raw (device).
+        A complex128 raw cube (with float64 amplitudes) runs the all-FP64
+        reference accumulator osar_echo_accumulate_fp64."""
+        import torch
+        fp64 = raw.dtype == torch.complex128
+        d = self.echo_desc(dx, dy, dz, snap0, amp if not fp64 else None, rcs0, pulse0, npulses,
+                           raw, shape, spec, spec_energy)
+        if fp64:
+            d.raw = None
+            d.raw64 = _lib.ptr(raw)
+            if self.sc.bragg_enable:
+                d.amp = None
+                d.amp64 = _lib.ptr(amp)
+                d.n_rcs_slots = int(amp.shape[0])
+                m, p = self.fp64_tables()
+                d.scat64_m, d.scat64_p = _lib.ptr(m), _lib.ptr(p)
+            _lib.check(self.ctx.lib.osar_echo_accumulate_fp64(self.ctx.h, ctypes.byref(d),
+                                                              _lib.stream_ptr()))
+            return
         _lib.check(self.ctx.lib.osar_echo_accumulate(self.ctx.h, ctypes.byref(d),
                                                      _lib.stream_ptr()))
 
@@ -401,7 +443,7 @@ class DeviceScene:
 
     # ------------------------------------------------------------------
     def run(self, p0=0, p1=None, chunk_snaps=32, raw=None, shape=None, timer=None,
-            overlap=False):
+            overlap=False, fp64=False):
         """GPU raw cube for pulses [p0, p1): returns (raw device tensor, NRCS_avg numpy).
 
         Snapshots are synthesised in chunks of `chunk_snaps` intervals, Bragg
@@ -410,14 +452,17 @@ class DeviceScene:
         Bragg of chunk c + 1 run on one stream while the accumulation of chunk
         c runs on another (double-buffered snapshots and amplitudes). `timer`,
         if given, is called as timer(stage, start_event, end_event) for every
-        launch group.
+        launch group. fp64: the all-FP64 reference path (FP64 synthesis
+        intermediates, FP64 Bragg facet model, osar_echo_accumulate_fp64) into a
+        complex128 raw cube.
         """
-        st = self.start(p0, p1, chunk_snaps, raw, shape, overlap=overlap)
+        st = self.start(p0, p1, chunk_snaps, raw, shape, overlap=overlap, fp64=fp64)
         while self.step(st, timer):
             pass
         return self.finish(st)
 
-    def start(self, p0=0, p1=None, chunk_snaps=32, raw=None, shape=None, overlap=False):
+    def start(self, p0=0, p1=None, chunk_snaps=32, raw=None, shape=None, overlap=False,
+              fp64=False):
         """Allocate the buffers of a run over pulses [p0, p1) (no launches)."""
         import torch
         sc = self.sc
@@ -426,7 +471,10 @@ class DeviceScene:
         st.p1 = self.n_pulses if p1 is None else int(p1)
         st.C = max(1, int(chunk_snaps))
         st.shape = shape
-        st.raw = self.new_raw() if raw is None else raw
+        st.fp64 = bool(fp64)
+        if fp64 and overlap:
+            raise ValueError('the FP64 reference path runs on one stream')
+        st.raw = self.new_raw(fp64) if raw is None else raw
         st.names = self.fields_needed()
         nx, ny = sc.Nx, sc.Ny
         nbuf = 2 if overlap else 1
@@ -436,7 +484,8 @@ class DeviceScene:
         st.rcs_sum = torch.zeros(max(nref, 1), self.npol, dtype=torch.float64, device=self.dev)
         st.amps = [None] * nbuf
         if sc.bragg_enable:
-            st.amps = [torch.empty((st.C + 2, self.npol, 2, nx, ny), dtype=torch.float32,
+            st.amps = [torch.empty((st.C + 2, self.npol, 2, nx, ny),
+                                   dtype=torch.float64 if fp64 else torch.float32,
                                    device=self.dev) for _ in range(nbuf)]
         st.k = 0                # buffer set of the next chunk
         st.held = -1            # refresh index whose amplitudes sit in slot held_slot
@@ -499,9 +548,10 @@ class DeviceScene:
             for n in st.names:
                 buf[n][0].copy_(src[n][st.snap_slot])
             self.synth(self.snap_times[m0 + 1:m_last + 1], {n: buf[n][1:] for n in st.names},
-                       st.names, self.slope_fp32)
+                       st.names, self.slope_fp32 and not st.fp64)
         else:
-            self.synth(self.snap_times[m0:m_last + 1], buf, st.names, self.slope_fp32)
+            self.synth(self.snap_times[m0:m_last + 1], buf, st.names,
+                       self.slope_fp32 and not st.fp64)
         st.snap_held, st.snap_slot = m_last, m_last - m0
         self._mark(timer, 'synth', ev)
         r0 = r1 = 0
@@ -521,7 +571,7 @@ class DeviceScene:
                 tmp = {n: torch.empty((1, sc.Nx, sc.Ny), dtype=torch.float32, device=self.dev)
                        for n in ('Diffx', 'Diffy')}
                 self.synth(self.snap_times[pr // K:pr // K + 1], tmp, ['Diffx', 'Diffy'],
-                           self.slope_fp32)
+                           self.slope_fp32 and not st.fp64)
                 self.bragg(tmp['Diffx'][0], tmp['Diffy'][0], pr, amp[0], st.rcs_sum[r0])
                 r_first += 1
             if r_first <= r1:

@@ -1,0 +1,156 @@
+"""GPU parity of the all-FP64 reference path, and the mixed-precision path
+measured against it over whole runs.
+
+The FP64 path (FP64 synthesis intermediates, FP64 Bragg facet model,
+osar_echo_accumulate_fp64) evaluates the oracle's arithmetic in FP64 on the
+GPU: it matches the FP64 numpy / C oracle to FP64 rounding (relative RMS below
+1e-12 on the oracle's own float32 fields, 1e-7 end to end, where the GPU's
+float32 fields round a few values within 1e-15 of a boundary the other way)
+on windows, which pins it, and then stands in for the oracle over a
+whole run, where the CPU cannot follow: the 2048^2, 4-channel, 8192-pulse
+workload's mixed-precision raw cube is compared with it end to end
+(relative RMS <= 1e-4, phase RMS <= 1 mrad, every channel).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_rms(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def phase_rms(a, b, floor_db=-30.):
+    mag = np.abs(b)
+    keep = mag > mag.max() * 10 ** (floor_db / 20.)
+    d = np.angle(a[keep] * np.conj(b[keep]))
+    return float(np.sqrt(np.mean(d ** 2)))
+
+
+def _scene(tmp_path_factory, name, **kw):
+    from oceansar_amd import configs
+    base = kw.pop('base', 'small')
+    return configs.load_scene(configs.workload(base, **kw), tmp_path_factory.mktemp(name))
+
+
+def test_fp64_bragg_kernel_matches_oracle(need_gpu, tmp_path_factory):
+    import torch
+    import oracle.sar as osar
+    import oracle.surface as osurf
+    from oceansar_amd.engine import DeviceScene
+    sc = _scene(tmp_path_factory, 'b64', L=128., spread='elfouhaily', pol='DP', wind_dir=60.)
+    ds = DeviceScene(sc)
+    m = 7
+    snap = osurf.synth(sc.surface, sc.snapshot_time(m), ('D', 'Diff'))
+    dev = {n: torch.as_tensor(np.ascontiguousarray(snap[n].T), device='cuda')
+           for n in ('Diffx', 'Diffy')}
+    amp = torch.empty((2, 2, sc.Nx, sc.Ny), dtype=torch.float64, device='cuda')
+    tot = torch.zeros(2, dtype=torch.float64, device='cuda')
+    ds.bragg(dev['Diffx'], dev['Diffy'], m * sc.K, amp, tot)
+    cache = osar.SnapshotCache(sc)
+    cache.cache[m] = snap
+    rcs = osar.bragg_rcs(sc, cache, m * sc.K)
+    for ip, pol in enumerate(sc.pols):
+        got = amp[ip].cpu().numpy().transpose(0, 2, 1)
+        ref = np.sqrt(rcs[pol])
+        keep = ref > 1e-12
+        err = np.max(np.abs(got[keep] - ref[keep]) / ref[keep])
+        print('FP64 Bragg %s: max rel err %.2e' % (pol, err))
+        assert err < 1e-12, pol
+        assert abs(tot[ip].item() / rcs[pol].sum() - 1) < 1e-12
+
+
+@pytest.mark.parametrize('case', ['plain', 'dp_xti', 'specular'])
+def test_fp64_path_matches_oracle(need_gpu, tmp_path_factory, case):
+    import torch
+    import oracle.sar as osar
+    from oceansar_amd.engine import DeviceScene
+    if case == 'plain':
+        sc = _scene(tmp_path_factory, 'f1')
+        p0, p1 = 450, 510
+    elif case == 'dp_xti':
+        sc = _scene(tmp_path_factory, 'f2', pol='DP', num_ch=3, b_ati=[0., 7., 15.],
+                    b_xti=[0., 30., -12.], spread='elfouhaily', K=4, wind_dir=30.)
+        p0, p1 = 500, 530
+    else:
+        sc = _scene(tmp_path_factory, 'f3', base='ati_swell', L=128., n_pulses=512, inc=8.)
+        p0, p1 = 250, 262
+    ds = DeviceScene(sc)
+    raw, nrcs = ds.run(p0, p1, chunk_snaps=2, fp64=True)
+    torch.cuda.synchronize()
+    assert raw.dtype == torch.complex128
+    got = raw[:, :, p0:p1].cpu().numpy()
+    ref, ref_nrcs = osar.simulate(sc, p0, p1)
+    e, ph = rel_rms(got, ref), phase_rms(got, ref)
+    # the surface fields are float32 (as the reference stores them): a value
+    # within 1e-15 of a float32 rounding boundary may round the other way,
+    # which moves that facet's phase by up to ~1e-5 rad; the rest is FP64
+    print('FP64 path %s: rel RMS %.2e phase RMS %.2e' % (case, e, ph))
+    assert e < 1e-7 and ph < 1e-7
+    np.testing.assert_allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-9)
+
+
+def test_fp64_accumulator_on_oracle_fields(need_gpu, tmp_path_factory):
+    """The FP64 accumulator fed the oracle's own float32 snapshots and the FP64
+    facet model's amplitudes: equal to the oracle to FP64 rounding."""
+    import torch
+    import oracle.sar as osar
+    import oracle.surface as osurf
+    from oceansar_amd.engine import DeviceScene
+    sc = _scene(tmp_path_factory, 'f4', pol='DP', num_ch=2, b_ati=[0., 10.], K=4)
+    p0, n = 392, 24
+    m0, m1 = p0 // sc.K, (p0 + n - 1) // sc.K + 1
+    snaps = [osurf.synth(sc.surface, sc.snapshot_time(m), ('D', 'Diff'))
+             for m in range(m0, m1 + 1)]
+    dev = {k: torch.as_tensor(np.ascontiguousarray(np.stack([s[k].T for s in snaps])),
+                              device='cuda') for k in ('Dx', 'Dy', 'Dz', 'Diffx', 'Diffy')}
+    ds = DeviceScene(sc)
+    r0, r1 = int(sc.rcs_slot[p0]), int(sc.rcs_slot[p0 + n - 1])
+    amp = torch.empty((r1 - r0 + 1, 2, 2, sc.Nx, sc.Ny), dtype=torch.float64, device='cuda')
+    for r in range(r0, r1 + 1):
+        pr = int(sc.rcs_refresh[r])
+        ds.bragg(dev['Diffx'][pr // sc.K - m0], dev['Diffy'][pr // sc.K - m0], pr, amp[r - r0])
+    raw = ds.new_raw(fp64=True)
+    ds.accumulate(dev['Dx'], dev['Dy'], dev['Dz'], m0, amp, r0, p0, n, raw)
+    torch.cuda.synchronize()
+    got = raw[:, :, p0:p0 + n].cpu().numpy()
+    cache = osar.SnapshotCache(sc, keep=len(snaps) + 1)
+    for i, m in enumerate(range(m0, m1 + 1)):
+        cache.cache[m] = snaps[i]
+    ref, _ = osar.simulate(sc, p0, p0 + n, snaps=cache)
+    e = rel_rms(got, ref)
+    print('FP64 accumulator on the oracle fields: rel RMS %.2e' % e)
+    assert e < 1e-12
+
+
+def test_large_workload_mixed_against_fp64_whole_run(need_gpu, tmp_path_factory):
+    """2048^2 x 4 channels x 8192 pulses: the whole mixed-precision raw cube
+    against the whole all-FP64 one, and a window of the FP64 run against the C
+    oracle."""
+    import torch
+    import oracle.c_oracle as co
+    import oracle.sar as osar
+    from oceansar_amd.engine import DeviceScene
+    sc = _scene(tmp_path_factory, 'L', base='large')
+    assert (sc.Nx, sc.num_ch, sc.az_steps) == (2048, 4, 8192)
+    ds = DeviceScene(sc)
+    mixed, n_mixed = ds.run(chunk_snaps=128)
+    mixed = mixed.cpu().numpy()
+    torch.cuda.empty_cache()
+    ref, n_ref = ds.run(chunk_snaps=64, fp64=True)
+    torch.cuda.synchronize()
+    ref = ref.cpu().numpy()
+    e, ph = rel_rms(mixed, ref), phase_rms(mixed, ref)
+    per_ch = [rel_rms(mixed[:, c], ref[:, c]) for c in range(sc.num_ch)]
+    print('2048^2 x 4 ch x 8192 pulses, mixed vs all-FP64 GPU: rel RMS %.3e phase RMS %.3e '
+          'rad; per channel %s' % (e, ph, ', '.join('%.2e' % x for x in per_ch)))
+    assert e < 1e-4 and ph < 1e-3 and max(per_ch) < 1e-4
+    np.testing.assert_allclose(n_mixed, n_ref, rtol=1e-6)
+    p0, p1 = 4088, 4104
+    co.build()
+    want, want_nrcs = co.simulate_c(sc, p0, p1, snaps=osar.SnapshotCache(sc))
+    e64 = rel_rms(ref[:, :, p0:p1], want)
+    print('all-FP64 GPU window [%d, %d) vs C oracle: rel RMS %.2e' % (p0, p1, e64))
+    assert e64 < 1e-7
+    np.testing.assert_allclose(n_ref[:, p0:p1], want_nrcs, rtol=1e-9)
