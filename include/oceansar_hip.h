@@ -229,6 +229,14 @@ int osar_fill_zero_f64(osar_ctx* ctx, double* p, int64_t n, void* stream);
 #define OSAR_F_DIFFXX 32
 #define OSAR_F_DIFFYY 64
 #define OSAR_F_DIFFXY 128
+#define OSAR_F_VX 256           /* orbital velocities (ocean.py:484-494) */
+#define OSAR_F_VY 512
+#define OSAR_F_VZ 1024
+#define OSAR_F_AX 2048          /* accelerations (ocean.py:496-506) */
+#define OSAR_F_AY 4096
+#define OSAR_F_AZ 8192
+#define OSAR_F_HMTF 16384       /* hydrodynamic modulation (ocean.py:508-526) */
+#define OSAR_NUM_FIELDS 15
 
 typedef struct osar_surface_desc {
   int32_t nx, ny;           /* powers of two, 16 .. 2048 */
@@ -255,8 +263,8 @@ typedef struct osar_surface_desc {
   const double* times;      /* [n_times] snapshot times */
   /* outputs: float32 [n_times][nx][ny] (x-major: field[x][y]) for every
      requested field, NULL otherwise */
-  float* out[8];            /* indexed by bit position: dx, dy, dz, diffx, diffy,
-                               diffxx, diffyy, diffxy */
+  float* out[OSAR_NUM_FIELDS]; /* indexed by bit position: dx, dy, dz, diffx, diffy,
+                               diffxx, diffyy, diffxy, vx, vy, vz, ax, ay, az, hmtf */
 } osar_surface_desc;
 
 /* Workspace bytes osar_surface_synth needs for this descriptor. */

@@ -56,11 +56,15 @@ class BraggDesc(ctypes.Structure):
         [('amp64', c_void_p)]
 
 
+NUM_FIELDS = 15   # OSAR_NUM_FIELDS: output slots of osar_surface_desc
+
+
 class SurfaceDesc(ctypes.Structure):
     _fields_ = [(n, c_int32) for n in ('nx', 'ny', 'fields', 'choppy', 'n_times',
                                        'mirror_grid', 'slope_fp32', 'reserved')] + \
         [(n, c_double) for n in ('depth', 'current_x', 'current_y', 'kx_res', 'ky_res')] + \
-        [(n, c_void_p) for n in ('kx', 'ky', 'coefs', 'times')] + [('out', c_void_p * 8)]
+        [(n, c_void_p) for n in ('kx', 'ky', 'coefs', 'times')] + \
+        [('out', c_void_p * NUM_FIELDS)]
 
 
 class FocusDesc(ctypes.Structure):
@@ -70,9 +74,9 @@ class FocusDesc(ctypes.Structure):
         [(n, c_void_p) for n in ('raw', 'fr', 'rcmc', 'h', 'work', 'slc')]
 
 
-FIELD_BITS = {'Dx': 1, 'Dy': 2, 'Dz': 4, 'Diffx': 8, 'Diffy': 16, 'Diffxx': 32,
-              'Diffyy': 64, 'Diffxy': 128}
-FIELD_ORDER = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
+FIELD_ORDER = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy',
+               'Vx', 'Vy', 'Vz', 'Ax', 'Ay', 'Az', 'hMTF']
+FIELD_BITS = {n: 1 << i for i, n in enumerate(FIELD_ORDER)}
 
 EXPORTS = ['osar_ctx_create', 'osar_ctx_destroy', 'osar_last_error', 'osar_version',
            'osar_struct_size', 'osar_echo_accumulate', 'osar_echo_accumulate_fp64',

@@ -49,6 +49,13 @@ enum Mult : int {
   M_NKXKY,   // -kx ky       (Diffxy)
   M_NKX2KI,  // -kx^2 / k    (choppy denominator x)
   M_NKY2KI,  // -ky^2 / k    (choppy denominator y)
+  M_WKXN,    // omega kx/k         (Vx: -Im ifft2(kx/k (-i omega) a))
+  M_WKYN,    // omega ky/k         (Vy)
+  M_NIW,     // -i omega           (Vz)
+  M_NIW2KXN, // -i omega^2 kx/k    (Ax)
+  M_NIW2KYN, // -i omega^2 ky/k    (Ay)
+  M_NW2,     // -omega^2           (Az)
+  M_HMTF,    // the hydrodynamic modulation transfer function M_h(k) (hMTF)
 };
 
 struct QPlan {
@@ -59,11 +66,16 @@ struct QPlan {
   int f32;         // 1: the row pass stores this spectrum as complex64 (else complex128)
   int cbw, lcbw;   // columns per contiguous run of a workspace row (128 bytes) and log2
   long long off;   // byte offset of this spectrum's plane within a snapshot's workspace
+  float* pa;       // output planes of outa / outb (NULL for -1)
+  float* pb;
 };
 
-// Every multiplier is a monomial s i^d kx^px ky^py |k|^-pk. Code bits: px (0-1),
-// py (2-3), pk (4), d (5), s = -1 (6), no multiplier (7).
-constexpr int kMonoImag = 32, kMonoNeg = 64, kMonoNone = 128;
+// Every multiplier but M_h is a monomial s i^d kx^px ky^py |k|^-pk omega^po.
+// Code bits: px (0-1), py (2-3), pk (4), d (5), s = -1 (6), no multiplier (7),
+// po (8-9); M_h (10) is even in k with complex values (see hmtf_mult).
+constexpr int kMonoImag = 32, kMonoNeg = 64, kMonoNone = 128, kMonoOmega = 256,
+              kMonoHmtf = 1024;
+constexpr int kNumFields = 15;
 
 struct SynthArgs {
   int nx, ny, nq, logx, logy, cb, lcb, n_times;
@@ -77,11 +89,10 @@ struct SynthArgs {
   unsigned char* work;  // workspace: per snapshot, the planes of the nq spectra
   long long snap_bytes; // workspace bytes per snapshot
   double* omega;        // [ny][nx] omega(|k|), filled by omega_kernel
-  float* out[8];
   QPlan q[kMaxQ];
 };
 
-// Plan entry / output pointer at a wave-uniform runtime index. A switch keeps
+// Plan entry at a wave-uniform runtime index. A switch keeps
 // the by-value kernel argument in scalar registers (indexing its arrays with a
 // runtime value would copy the whole argument to scratch memory).
 __device__ __forceinline__ QPlan plan_at(const SynthArgs& a, int i) {
@@ -94,19 +105,6 @@ __device__ __forceinline__ QPlan plan_at(const SynthArgs& a, int i) {
     case 5: return a.q[5];
     case 6: return a.q[6];
     default: return a.q[7];
-  }
-}
-
-__device__ __forceinline__ float* out_at(const SynthArgs& a, int i) {
-  switch (i) {
-    case 0: return a.out[0];
-    case 1: return a.out[1];
-    case 2: return a.out[2];
-    case 3: return a.out[3];
-    case 4: return a.out[4];
-    case 5: return a.out[5];
-    case 6: return a.out[6];
-    default: return a.out[7];
   }
 }
 
@@ -123,12 +121,40 @@ __device__ __forceinline__ double mono_y(int code, double ky) {
   return (code & kMonoNeg) ? -y : y;
 }
 
-__device__ __forceinline__ double mono_x(int code, double kx, double ik) {
+__device__ __forceinline__ double mono_x(int code, double kx, double ik, double om) {
   const int px = code & 3;
   double f = px >= 1 ? kx : 1.;
   if (px == 2) f = f * kx;
   if (code & 16) f = f * ik;
+  const int po = (code >> 8) & 3;
+  if (po >= 1) f = f * om;
+  if (po == 2) f = f * om;
   return f;
+}
+
+// The hydrodynamic MTF of ocean.py:508-526: M_h = i k omega (omega + i mu) /
+// (omega^2 + mu^2) (dlnS/dlnk - Vg/c), mu = 0.04 / 16, dlnS/dlnk = -3, Vg/c =
+// 0.5, zero for k > 2 pi / 20; even in k (M_h(-k) = M_h(k)). The cut compares
+// |k| as the reference's float32 table holds it (sqrt(kx^2 + ky^2), each
+// operation rounded to float32) with 2 pi / 20 rounded to float32, so the same
+// wave numbers are cut; the value is FP64.
+__device__ __forceinline__ double2 hmtf_mult(float kx, float ky, double om) {
+  constexpr double kMu = 0.04 / 16., kFac = -3. - 0.5;
+  constexpr float kMax32 = (float)(2. * M_PI / 20.);
+  const float k32 = __fsqrt_rn(__fadd_rn(__fmul_rn(kx, kx), __fmul_rn(ky, ky)));
+  if (k32 > kMax32) return make_double2(0., 0.);
+  const double k = sqrt((double)kx * kx + (double)ky * ky);
+  const double den = om * om + kMu * kMu;
+  // i k om (om + i mu) = -k om mu + i k om^2
+  return make_double2((-k * om * kMu) / den * kFac, (k * om * om) / den * kFac);
+}
+
+// A monomial m with m(-k) = -conj(m(k)) (kx^px ky^py i^d of odd px + py + d:
+// the velocities' omega k/|k| and -i omega): then H(m a) = m G(a) with
+// G(a) = (a(k) - conj(a(-k))) / 2, not m H(a).
+__device__ __forceinline__ bool mono_anti(int code) {
+  if (code & (kMonoNone | kMonoHmtf)) return false;
+  return (((code & 3) + ((code >> 2) & 3) + ((code & kMonoImag) ? 1 : 0)) & 1) != 0;
 }
 
 // the multiplier value: v real, or i v when d
@@ -200,9 +226,12 @@ __device__ __forceinline__ void phased_pair(const SynthArgs& a, int j, int i, in
 // the FFT buffer lives in LDS (2 rows, 64 KB at 2048), so two workgroups share
 // a CU and one's HBM stores overlap the other's FFT.
 // Away from the Nyquist row and column, -k is the true mirror, every
-// multiplier has m(-k) = conj(m(k)) and H(m a) = m H(a): the thread keeps only
-// H(a) = (a(k) + conj(a(-k))) / 2 per column pair in registers. The Nyquist
-// column's pair (one per row pair) is kept in LDS by its thread.
+// multiplier of the raw-data fields has m(-k) = conj(m(k)) and H(m a) = m H(a):
+// the thread keeps only H(a) = (a(k) + conj(a(-k))) / 2 per column pair in
+// registers. The velocities' multipliers have m(-k) = -conj(m(k)) and M_h is
+// even in k; those spectra (TIMED) also need G(a) = (a(k) - conj(a(-k))) / 2,
+// recomputed from the coefficients. The Nyquist column's pair (one per row
+// pair) is kept in LDS by its thread.
 // MIRROR (the tables are a 2 pi fftfreq grid: kx(-i) = -kx(i) off the Nyquist
 // column, kx(0) = 0, the same for ky): the only elements that need the general
 // form are the Nyquist column and the self-paired rows 0 and ny / 2, and a
@@ -212,7 +241,9 @@ __device__ __forceinline__ void phased_pair(const SynthArgs& a, int j, int i, in
 // and a(-k) where it needs them.
 // Rows go to the workspace in column-blocked layout W[q][x / cb][y][x % cb],
 // so that pass 2 reads each block of cb columns as one contiguous run.
-template <bool SHORT, bool MIRROR>
+// TIMED: some spectrum's multiplier involves omega (velocities, accelerations)
+// or is M_h; the rest compile without those branches and their registers.
+template <bool SHORT, bool MIRROR, bool TIMED>
 __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(SynthArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double2* S = (double2*)lds_raw;  // [2][nx] packed spectrum of rows j, jm
@@ -258,6 +289,11 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
     const double ya = mono_y(q.ca, kyj), yb = mono_y(q.cb, kyj);
     const bool da = (q.ca & kMonoImag) != 0, db = (q.cb & kMonoImag) != 0;
     const bool need_ik = ((q.ca | q.cb) & 16) != 0;   // a multiplier divides by |k|
+    const bool need_om = TIMED && ((q.ca | q.cb) & (3 * kMonoOmega)) != 0;
+    const bool hmtf = TIMED && (q.ca & kMonoHmtf) != 0;   // M_h, alone in its spectrum
+    const bool anti_a = TIMED && mono_anti(q.ca), anti_b = TIMED && mono_anti(q.cb);
+    // the spectrum needs G(a) besides H(a)
+    const bool need_g = hmtf || anti_a || anti_b;
     // Opaque per iteration: the index and address arithmetic of the build,
     // the FFT and the write-out is recomputed per spectrum instead of being
     // held in registers across all of them.
@@ -282,14 +318,39 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
           const double k = sqrt(kx * kx + (double)kyj * (double)kyj);
           ik = k > a.thr ? 1. / k : 0.;
         }
-        const double fa = mono_x(q.ca, kx, ik) * ya, fb = mono_x(q.cb, kx, ik) * yb;
-        const double ar = da ? 0. : fa, ai = da ? fa : 0.;
-        const double br = db ? 0. : fb, bi = db ? fb : 0.;
-        if (in) {
-          S[pad(i)] = cmul2(make_double2(ar - bi, ai + br), h);
-          // for a self-paired row this rewrites the value the mirror's thread writes
-          S[pad((nrow - 1) * nx + im)] =
-              cmul2(make_double2(ar + bi, br - ai), make_double2(h.x, -h.y));
+        // omega(|k|) = omega(|-k|): from the dispersion table when a multiplier needs it
+        const double om = (need_om || hmtf) ? a.omega[(size_t)j * nx + i] : 0.;
+        if (need_g) {
+          // G(a) from a(k) and a(-k), recomputed; the thread of the mirror
+          // column of a self-paired row computes -conj(G) to the bit
+          double2 A0, A1;
+          phased_pair<MIRROR>(a, j, i, jm, im, t, A0, A1);
+          const double2 g = make_double2(0.5 * (A0.x - A1.x), 0.5 * (A0.y + A1.y));
+          double2 ha, hb;
+          if (hmtf) {
+            // M_h(-k) = M_h(k): H(M_h a) = Re(M_h) H(a) + i Im(M_h) G(a)
+            const double2 M = hmtf_mult((float)kx, kyj, om);
+            ha = make_double2(M.x * h.x - M.y * g.y, M.x * h.y + M.y * g.x);
+            hb = make_double2(0., 0.);
+          } else {
+            ha = cmul2(mono_c(da, mono_x(q.ca, kx, ik, om) * ya), anti_a ? g : h);
+            hb = cmul2(mono_c(db, mono_x(q.cb, kx, ik, om) * yb), anti_b ? g : h);
+          }
+          if (in) {
+            // ha + i hb at k, conj(ha) + i conj(hb) at -k
+            S[pad(i)] = make_double2(ha.x - hb.y, ha.y + hb.x);
+            S[pad((nrow - 1) * nx + im)] = make_double2(ha.x + hb.y, hb.x - ha.y);
+          }
+        } else {
+          const double fa = mono_x(q.ca, kx, ik, om) * ya, fb = mono_x(q.cb, kx, ik, om) * yb;
+          const double ar = da ? 0. : fa, ai = da ? fa : 0.;
+          const double br = db ? 0. : fb, bi = db ? fb : 0.;
+          if (in) {
+            S[pad(i)] = cmul2(make_double2(ar - bi, ai + br), h);
+            // for a self-paired row this rewrites the value the mirror's thread writes
+            S[pad((nrow - 1) * nx + im)] =
+                cmul2(make_double2(ar + bi, br - ai), make_double2(h.x, -h.y));
+          }
         }
       } else if (in) {
         general |= 1u << u;
@@ -325,11 +386,15 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
       const double ik = k0 > a.thr ? 1. / k0 : 0.;
       const double km = sqrt(kxm * kxm + kym * kym);
       const double ikm = km > a.thr ? 1. / km : 0.;
+      const double om0 = (need_om || hmtf) ? a.omega[(size_t)j * nx + i] : 0.;
+      const double omm = (need_om || hmtf) ? a.omega[(size_t)jm * nx + im] : 0.;
       const double yam = mono_y(q.ca, kym), ybm = mono_y(q.cb, kym);
-      const double2 mak = mono_c(da, mono_x(q.ca, kx, ik) * ya);
-      const double2 mbk = mono_c(db, mono_x(q.cb, kx, ik) * yb);
-      const double2 mam = mono_c(da, mono_x(q.ca, kxm, ikm) * yam);
-      const double2 mbm = mono_c(db, mono_x(q.cb, kxm, ikm) * ybm);
+      const double2 mak = hmtf ? hmtf_mult((float)kx, (float)ky, om0)
+                               : mono_c(da, mono_x(q.ca, kx, ik, om0) * ya);
+      const double2 mbk = mono_c(db, mono_x(q.cb, kx, ik, om0) * yb);
+      const double2 mam = hmtf ? hmtf_mult((float)kxm, (float)kym, omm)
+                               : mono_c(da, mono_x(q.ca, kxm, ikm, omm) * yam);
+      const double2 mbm = mono_c(db, mono_x(q.cb, kxm, ikm, omm) * ybm);
       const double2 ha = herm(mak, A0, mam, A1);
       const double2 hb = herm(mbk, A0, mbm, A1);
       S[pad(i)] = make_double2(ha.x - hb.y, ha.y + hb.x);  // ha + i hb at k
@@ -414,8 +479,8 @@ __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(Synt
   const double scale = 1.0 / ((double)nx * (double)ny);
   const double t = a.times[ti];
   const size_t plane = (size_t)nx * ny;
-  float* const outa = q.outa >= 0 ? out_at(a, q.outa) : nullptr;
-  float* const outb = q.outb >= 0 ? out_at(a, q.outb) : nullptr;
+  float* const outa = q.pa;
+  float* const outb = q.pb;
   for (int e = tid; e < ny * cb; e += kColThreads) {
     const int c = e >> a.logy, y = e & (ny - 1);  // consecutive threads walk y: contiguous stores
     const double2 z = C[pad(e)];
@@ -427,14 +492,18 @@ __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(Synt
     }
     if (q.outa >= 0) {
       double v = re;
-      if (q.outa == 0) v = v + a.cur_x * t;
+      if (q.outa == 0) v = v + a.cur_x * t;   // Dx, Dy drift with the current
       if (q.outa == 1) v = v + a.cur_y * t;
+      if (q.outa == 8) v = v + a.cur_x;       // Vx, Vy carry it
+      if (q.outa == 9) v = v + a.cur_y;
       __builtin_nontemporal_store((float)v, &outa[o]);
     }
     if (q.outb >= 0) {
       double v = im;
       if (q.outb == 0) v = v + a.cur_x * t;
       if (q.outb == 1) v = v + a.cur_y * t;
+      if (q.outb == 8) v = v + a.cur_x;
+      if (q.outb == 9) v = v + a.cur_y;
       __builtin_nontemporal_store((float)v, &outb[o]);
     }
   }
@@ -447,8 +516,9 @@ int ilog2(int n) {
 
 // Monomial code of a multiplier (QPlan::ca / cb).
 int mono_code(int m) {
-  auto c = [](int px, int py, int pk, int imag, int neg) {
-    return px | (py << 2) | (pk << 4) | (imag ? kMonoImag : 0) | (neg ? kMonoNeg : 0);
+  auto c = [](int px, int py, int pk, int imag, int neg, int po = 0) {
+    return px | (py << 2) | (pk << 4) | (imag ? kMonoImag : 0) | (neg ? kMonoNeg : 0) |
+           po * kMonoOmega;
   };
   switch (m) {
     case M_ONE: return c(0, 0, 0, 0, 0);
@@ -461,22 +531,39 @@ int mono_code(int m) {
     case M_NKXKY: return c(1, 1, 0, 0, 1);
     case M_NKX2KI: return c(2, 0, 1, 0, 1);
     case M_NKY2KI: return c(0, 2, 1, 0, 1);
+    case M_WKXN: return c(1, 0, 1, 0, 0, 1);
+    case M_WKYN: return c(0, 1, 1, 0, 0, 1);
+    case M_NIW: return c(0, 0, 0, 1, 1, 1);
+    case M_NIW2KXN: return c(1, 0, 1, 1, 1, 2);
+    case M_NIW2KYN: return c(0, 1, 1, 1, 1, 2);
+    case M_NW2: return c(0, 0, 0, 0, 1, 2);
+    case M_HMTF: return kMonoHmtf;
     default: return kMonoNone;
   }
 }
 
-int make_plan(const osar_surface_desc* d, QPlan* q, int* nq) {
-  // Real outputs in pairing order with their multipliers.
+int make_plan(const osar_surface_desc* d, QPlan* q_out, int* nq) {
+  // Real outputs in pairing order with their multipliers. At most 8 spectra
+  // result (14 monomial fields pair into 7, M_h takes one; choppy slopes take
+  // one each but exclude the 3 curvatures); q has room for twice that.
   struct Item { int out, m; };
-  static const int kMult[8] = {M_IKXN, M_IKYN, M_ONE, M_IKX, M_IKY, M_NKX2, M_NKY2, M_NKXKY};
-  static const int kOrder[8] = {2, 0, 1, 3, 4, 5, 6, 7};  // Dz, Dx, Dy, Diffx, Diffy, ...
+  QPlan q[2 * kMaxQ];
+  static const int kMult[kNumFields] = {M_IKXN, M_IKYN, M_ONE, M_IKX, M_IKY, M_NKX2, M_NKY2,
+                                        M_NKXKY, M_WKXN, M_WKYN, M_NIW, M_NIW2KXN, M_NIW2KYN,
+                                        M_NW2, M_HMTF};
+  // Dz, Dx, Dy, Diffx, Diffy, Diffxx, Diffyy, Diffxy, Vx, Vy, Vz, Ax, Ay, Az
+  static const int kOrder[kNumFields - 1] = {2, 0, 1, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13};
   std::vector<Item> items;
   int n = 0;
+  if (d->fields & ~((1 << kNumFields) - 1)) {
+    osar_set_error("unknown field bits 0x%x", d->fields & ~((1 << kNumFields) - 1));
+    return OSAR_EINVAL;
+  }
   if (d->choppy && (d->fields & (OSAR_F_DIFFXX | OSAR_F_DIFFYY | OSAR_F_DIFFXY))) {
     osar_set_error("choppy second derivatives are not supported by osar_surface_synth");
     return OSAR_EUNSUPPORTED;
   }
-  for (int oi = 0; oi < 8; ++oi) {
+  for (int oi = 0; oi < kNumFields - 1; ++oi) {
     const int f = kOrder[oi];
     if (!(d->fields & (1 << f))) continue;
     if (d->choppy && (f == 3 || f == 4)) {
@@ -485,6 +572,8 @@ int make_plan(const osar_surface_desc* d, QPlan* q, int* nq) {
     }
     items.push_back(Item{f, kMult[f]});
   }
+  // M_h is not a monomial: its spectrum takes it alone (component a)
+  if (d->fields & OSAR_F_HMTF) q[n++] = QPlan{M_HMTF, M_NONE, 14, -1, 0, 0, 0};
   // slope_fp32: Diffx and Diffy share one complex64 spectrum, the other fields
   // pair up in complex128 spectra
   std::vector<Item> lo;
@@ -503,12 +592,31 @@ int make_plan(const osar_surface_desc* d, QPlan* q, int* nq) {
     q[n++] = QPlan{lo[0].m, two ? lo[1].m : (int)M_NONE, lo[0].out, two ? lo[1].out : -1, 0,
                    0, 0, 1, 0, 0, 0};
   }
+  if (n > kMaxQ) {
+    osar_set_error("%d spectra exceed the %d a synthesis call takes", n, kMaxQ);
+    return OSAR_EUNSUPPORTED;
+  }
   for (int i = 0; i < n; ++i) {
     q[i].ca = mono_code(q[i].ma);
     q[i].cb = mono_code(q[i].mb);
+    q_out[i] = q[i];
   }
   *nq = n;
   return OSAR_OK;
+}
+
+// The row-pass variant: SHORT (padded addressing for nx >= 64), MIRROR, TIMED.
+const void* rows_kernel(bool short_x, bool mirror, bool timed) {
+  static const void* const k[8] = {
+      (const void*)synth_rows_kernel<false, false, false>,
+      (const void*)synth_rows_kernel<false, false, true>,
+      (const void*)synth_rows_kernel<false, true, false>,
+      (const void*)synth_rows_kernel<false, true, true>,
+      (const void*)synth_rows_kernel<true, false, false>,
+      (const void*)synth_rows_kernel<true, false, true>,
+      (const void*)synth_rows_kernel<true, true, false>,
+      (const void*)synth_rows_kernel<true, true, true>};
+  return k[(short_x ? 4 : 0) + (mirror ? 2 : 0) + (timed ? 1 : 0)];
 }
 
 // Column block (columns per column-pass workgroup) of an ny-point transform.
@@ -631,11 +739,14 @@ int surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream, unsig
   SynthArgs a{};
   OSAR_REQUIRE(make_plan(d, a.q, &a.nq) == OSAR_OK, OSAR_EUNSUPPORTED, "%s", osar_last_error());
   if (a.nq == 0) return OSAR_OK;
-  for (int f = 0; f < 8; ++f) {
+  for (int f = 0; f < kNumFields; ++f) {
     if (d->fields & (1 << f)) {
       OSAR_REQUIRE(d->out[f] != nullptr, OSAR_EINVAL, "output field %d requested but NULL", f);
     }
-    a.out[f] = d->out[f];
+  }
+  for (int i = 0; i < a.nq; ++i) {
+    a.q[i].pa = a.q[i].outa >= 0 ? d->out[a.q[i].outa] : nullptr;
+    a.q[i].pb = a.q[i].outb >= 0 ? d->out[a.q[i].outb] : nullptr;
   }
   OSAR_HIP(hipSetDevice(ctx->device));
   const size_t need = (size_t)osar_surface_workspace_bytes(d);
@@ -682,10 +793,10 @@ int surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream, unsig
   // short padded addressing for transforms of 64 points or more
   const bool short_x = d->nx >= 64, short_y = d->ny >= 64;
   const bool mirror = d->mirror_grid != 0;
-  const void* rows_k = short_x ? (mirror ? (const void*)synth_rows_kernel<true, true>
-                                         : (const void*)synth_rows_kernel<true, false>)
-                               : (mirror ? (const void*)synth_rows_kernel<false, true>
-                                         : (const void*)synth_rows_kernel<false, false>);
+  bool timed = false;
+  for (int i = 0; i < a.nq; ++i)
+    timed = timed || (((a.q[i].ca | a.q[i].cb) & (3 * kMonoOmega | kMonoHmtf)) != 0);
+  const void* rows_k = rows_kernel(short_x, mirror, timed);
   const void* cols_k = short_y ? (const void*)synth_cols_kernel<true>
                                : (const void*)synth_cols_kernel<false>;
   {
@@ -702,18 +813,10 @@ int surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream, unsig
     OSAR_LAUNCHED("omega_kernel");
   }
   const dim3 grid_rows((d->ny / 2 + 1) * d->n_times), grid_cols((d->nx / cb) * a.nq * d->n_times);
-  if (short_x && mirror)
-    hipLaunchKernelGGL((synth_rows_kernel<true, true>), grid_rows, dim3(kRowThreads), lds_rows, s,
-                       a);
-  else if (short_x)
-    hipLaunchKernelGGL((synth_rows_kernel<true, false>), grid_rows, dim3(kRowThreads), lds_rows,
-                       s, a);
-  else if (mirror)
-    hipLaunchKernelGGL((synth_rows_kernel<false, true>), grid_rows, dim3(kRowThreads), lds_rows,
-                       s, a);
-  else
-    hipLaunchKernelGGL((synth_rows_kernel<false, false>), grid_rows, dim3(kRowThreads), lds_rows,
-                       s, a);
+  {
+    void* args[] = {&a};
+    OSAR_HIP(hipLaunchKernel(rows_k, grid_rows, dim3(kRowThreads), args, lds_rows, s));
+  }
   OSAR_LAUNCHED("synth_rows_kernel");
   if (short_y)
     hipLaunchKernelGGL(synth_cols_kernel<true>, grid_cols, dim3(kColThreads), lds_cols, s, a);

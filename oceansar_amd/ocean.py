@@ -5,12 +5,13 @@ random coefficients (host, surface_init.py), load()/save() use the ocean
 state file, and setting ``t`` synthesises the requested fields through
 osar_surface_synth (FP64 FFTs, float32 results). After ``surface.t = t`` the
 fields are available as float32 numpy arrays [Ny, Nx] under the reference's
-attribute names (Dx, Dy, Dz, Diffx, Diffy, Diffxx, Diffyy, Diffxy) and as
-device tensors in x-major layout in ``surface.device_fields``.
+attribute names (Dx, Dy, Dz, Diffx, Diffy, Diffxx, Diffyy, Diffxy, Vx, Vy, Vz,
+Ax, Ay, Az, hMTF) and as device tensors in x-major layout in
+``surface.device_fields``.
 
-Supported compute flags: 'D', 'Diff', 'Diff2' (choppy 'Diff2' is not).
-'V', 'A' and 'hMTF' are outside the raw-data path and raise
-NotImplementedError.
+Supported compute flags: 'D', 'Diff', 'Diff2', 'V', 'A', 'hMTF' (choppy
+'Diff2' is not); every requested field of one time comes from one
+osar_surface_synth call, two fields per FP64 inverse FFT.
 """
 import numpy as np
 
@@ -18,7 +19,8 @@ from . import raw_io
 from .surface_init import SurfaceInit, init_surface
 
 _COMPUTE = {'D': ['Dx', 'Dy', 'Dz'], 'Diff': ['Diffx', 'Diffy'],
-            'Diff2': ['Diffxx', 'Diffyy', 'Diffxy']}
+            'Diff2': ['Diffxx', 'Diffyy', 'Diffxy'], 'V': ['Vx', 'Vy', 'Vz'],
+            'A': ['Ax', 'Ay', 'Az'], 'hMTF': ['hMTF']}
 
 
 class OceanSurface:

@@ -1,9 +1,13 @@
 """FP64 restatement of the OceanSurface time evolution (oceansar/surfaces/ocean.py:418-526).
 
 Differences from the reference, all towards FP64: t, omega(k), the
-dispersion phase and the inverse FFTs are complex128 (numpy >= 2 runs the
-reference's ifft2 in complex64; numpy < 2 in complex128). Outputs are stored
-as float32, as OceanSurface stores them (ocean.py:384-403).
+dispersion phase, the hMTF multiplier M_h and the inverse FFTs are complex128
+(numpy >= 2 runs the reference's ifft2 in complex64; numpy < 2 in
+complex128). The hMTF cut k > 2 pi / 20 compares |k| as the reference's
+float32 table holds it with 2 pi / 20 rounded to float32 (ocean.py:519-524
+compares float32 arrays with a Python float), so the same wave numbers are
+cut. Outputs are stored as float32, as OceanSurface stores them
+(ocean.py:384-403), or as float64 with dtype=np.float64 (tests only).
 """
 import numpy as np
 
@@ -31,8 +35,23 @@ def _re(a):
     return np.real(np.fft.ifft2(a))
 
 
-def synth(surf, t, compute=('D', 'Diff')):
-    """Fields at time t as a dict of float32 [Ny, Nx] arrays."""
+def hmtf_multiplier(surf, k, omega):
+    """M_h of ocean.py:511-524 over the grid (FP64), zero where the float32
+    |k| exceeds float32(2 pi / 20)."""
+    k_max = 2. * np.pi / 20.
+    mu = 0.04 / 16.
+    dlnS_dlnk = -3.
+    Vg_c = 0.5
+    M = 1j * k * omega * (omega + 1j * mu) / (omega ** 2. + mu ** 2.) * (dlnS_dlnk - Vg_c)
+    kx32 = surf.kx_s.astype(np.float32)[None, :]
+    ky32 = surf.ky_s.astype(np.float32)[:, None]
+    k32 = np.sqrt(kx32 ** 2 + ky32 ** 2)
+    return np.where(k32 > np.float32(k_max), 0., M)
+
+
+def synth(surf, t, compute=('D', 'Diff'), dtype=np.float32):
+    """Fields at time t as a dict of [Ny, Nx] arrays of `dtype` (float32 as
+    OceanSurface stores them)."""
     t = float(t)
     kx, ky, k, kxn, kyn, kinv, omega = wave_tables(surf)
     ph = surf.wave_coefs.astype(np.complex128) * np.exp(-1j * omega * t)
@@ -112,4 +131,7 @@ def synth(surf, t, compute=('D', 'Diff')):
             Ay = Ay + np.real(1j * sky / sk * sa)
             Az = Az + np.real(sa)
         out.update(Ax=Ax, Ay=Ay, Az=Az)
-    return {n: np.ascontiguousarray(v, dtype=np.float32) for n, v in out.items()}
+    if 'hMTF' in compute:
+        # monochromatic swell does not enter (ocean.py:508-526)
+        out['hMTF'] = _re(hmtf_multiplier(surf, k, omega) * ph)
+    return {n: np.ascontiguousarray(v, dtype=dtype) for n, v in out.items()}

@@ -103,6 +103,31 @@ def test_synth_matches_oracle_rectangular_choppy(need_gpu, tmp_path_factory):
     _synth_compare(sc, [0.0, 1.7], ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy'], ('D', 'Diff'))
 
 
+ALL_FIELDS = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy', 'Vx', 'Vy',
+              'Vz', 'Ax', 'Ay', 'Az', 'hMTF']
+
+
+def test_synth_all_fields_match_oracle(need_gpu, tmp_path_factory):
+    """Every OceanSurface field in one call: the 14 monomial fields pair into 7
+    spectra, the hydrodynamic MTF (a complex multiplier even in k) takes an
+    eighth."""
+    sc = _scene(tmp_path_factory, 'sall')
+    _synth_compare(sc, [0.0, 0.3711, 4.25], ALL_FIELDS, ('D', 'Diff', 'Diff2', 'V', 'A', 'hMTF'))
+
+
+def test_synth_time_derivatives_rectangular_choppy_current(need_gpu, tmp_path_factory):
+    """Velocities, accelerations and hMTF next to choppy slopes, Nx != Ny, a
+    current (Vx, Vy carry it) and finite depth."""
+    from oceansar_amd import configs
+    text = configs.workload('small', L=256., dx=1., choppy=True)
+    text = text.replace('Ly=256.0', 'Ly=128.0').replace('current_mag=0', 'current_mag=0.4')
+    text = text.replace('current_dir=90.', 'current_dir=30.').replace('depth=1000', 'depth=20')
+    sc = configs.load_scene(text, tmp_path_factory.mktemp('sva'))
+    assert (sc.Nx, sc.Ny) == (256, 128) and sc.surface.choppy
+    names = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Vx', 'Vy', 'Vz', 'Ax', 'Ay', 'Az', 'hMTF']
+    _synth_compare(sc, [0.0, 2.9], names, ('D', 'Diff', 'V', 'A', 'hMTF'))
+
+
 @pytest.mark.parametrize('lx,ly', [(16., 32.), (32., 16.)])
 def test_synth_matches_oracle_16_point_transforms(need_gpu, tmp_path_factory, lx, ly):
     """16-point row or column transforms: the padded-address fallback of the
@@ -126,7 +151,7 @@ def test_synth_general_row_pass_matches_mirror_pass(need_gpu, tmp_path_factory, 
     from oceansar_amd.engine import DeviceScene, synth_fields
     sc = configs.load_scene(configs.workload('small', L=lx, dx=1.), tmp_path_factory.mktemp('sg'))
     ds = DeviceScene(sc)
-    names = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
+    names = ALL_FIELDS
     res = []
     for mirror in (True, False):
         out = {n: torch.empty((2, sc.Nx, sc.Ny), dtype=torch.float32, device='cuda')

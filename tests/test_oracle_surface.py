@@ -90,3 +90,82 @@ def test_parseval_normalisation():
     h = 0.5 * (cc + np.conj(ci))
     assert np.isclose(np.sum(f['Dz'].astype(float) ** 2), np.sum(np.abs(h) ** 2) / (Nx * Ny),
                       rtol=1e-6)
+
+
+def _random_surf(Nx=64, Ny=32, dx=1.0, dy=1.5, depth=None, current=(0.3, -0.2), seed=5):
+    rng = np.random.default_rng(seed)
+    c = (rng.standard_normal((Ny, Nx)) + 1j * rng.standard_normal((Ny, Nx)))
+    return _surf(Nx, Ny, dx, dy, c, depth=depth, current=current)
+
+
+@pytest.mark.parametrize('depth', [None, 12.0])
+def test_velocities_and_accelerations_are_time_derivatives(depth):
+    """V = dD/dt and A = dV/dt (central differences of the FP64 fields): fixes
+    the omega powers, the i factors and signs, the direction cosines and the
+    current (Dx carries current * t, Vx carries current, Ax none)."""
+    s = _random_surf(depth=depth)
+    t, h = 3.3, 1e-4
+    f = lambda tt: osurf.synth(s, tt, ('D', 'V', 'A'), dtype=np.float64)
+    f0, fp, fm = f(t), f(t + h), f(t - h)
+    for d, v in (('Dx', 'Vx'), ('Dy', 'Vy'), ('Dz', 'Vz'), ('Vx', 'Ax'), ('Vy', 'Ay'),
+                 ('Vz', 'Az')):
+        deriv = (fp[d] - fm[d]) / (2 * h)
+        scale = np.sqrt(np.mean(f0[v] ** 2))
+        assert np.max(np.abs(deriv - f0[v])) < 1e-6 * scale, (d, v)
+    # the current (float32, as OceanSurface holds it): the mean of Vx, since
+    # the k = 0 coefficient has omega = 0 and kx / k = 0
+    assert abs(np.mean(f0['Vx']) - float(np.float32(0.3))) < 1e-12
+    assert abs(np.mean(f0['Ax'])) < 1e-12
+
+
+def test_hmtf_single_mode_closed_form():
+    """One coefficient at k below 2 pi / 20: hMTF = Re(-3.5 i k w z / (w - i mu)),
+    mu = 0.04 / 16, i.e. close to -3.5 times the slope along the wave vector
+    (a down-wind slope of a down-wind wave has positive modulation sign -3.5 *
+    d(eta)/d(x_k)); every coefficient beyond the cut gives zero."""
+    Nx, Ny, dx, dy = 64, 64, 4.0, 4.0
+    t = 1.7
+    n = np.arange(Nx)[None, :]
+    m = np.arange(Ny)[:, None]
+    for (i0, j0) in ((3, 0), (2, -3), (-4, 1)):
+        i0, j0 = i0 % Nx, j0 % Ny
+        c = np.zeros((Ny, Nx), complex)
+        c[j0, i0] = 2.0 + 0.7j
+        s = _surf(Nx, Ny, dx, dy, c)
+        kx, ky = float(s.kx_s[i0]), float(s.ky_s[j0])
+        k = np.hypot(kx, ky)
+        assert k < 2 * np.pi / 20
+        w = np.sqrt(G * k)
+        z = (complex(np.complex64(c[j0, i0])) * np.exp(-1j * w * t) / (Nx * Ny) *
+             np.exp(2j * np.pi * (i0 * n / Nx + j0 * m / Ny)))
+        f = osurf.synth(s, t, ('Diff', 'hMTF'), dtype=np.float64)
+        want = np.real(-3.5j * k * w / (w - 1j * 0.04 / 16) * z)
+        assert np.max(np.abs(f['hMTF'] - want)) < 1e-12 * np.max(np.abs(want))
+        slope_k = (kx * f['Diffx'] + ky * f['Diffy']) / k
+        # -3.5 slope_k to first order in mu / w (the relaxation shifts the phase by ~mu / w)
+        dev = np.max(np.abs(f['hMTF'] + 3.5 * slope_k)) / np.max(np.abs(want))
+        assert 0.95 * 0.04 / 16 / w < dev < 1.05 * 0.04 / 16 / w
+    c = np.zeros((Ny, Nx), complex)
+    c[0, 6] = c[5, 0] = c[4, 4] = 1.0   # |k| = 0.147, 0.123, 0.139 ... below; 20 is beyond
+    c[0, 20] = c[3, 19] = 5.0
+    s = _surf(Nx, Ny, dx, dy, c)
+    only_low = c.copy()
+    only_low[0, 20] = only_low[3, 19] = 0.
+    a = osurf.synth(s, t, ('hMTF',), dtype=np.float64)['hMTF']
+    b = osurf.synth(_surf(Nx, Ny, dx, dy, only_low), t, ('hMTF',), dtype=np.float64)['hMTF']
+    assert np.array_equal(a, b)
+
+
+def test_hmtf_cut_compares_float32_wave_numbers():
+    """At dx = 5 m, Nx = 64 the wave number of column 16 is 2 pi / 20 rounded
+    to float32, as is the cut: the reference's float32 comparison keeps it
+    (the FP64 value of the float32 number is above 2 pi / 20), column 17 is cut."""
+    Nx, Ny, dx, dy = 64, 16, 5.0, 5.0
+    for col, kept in ((16, True), (17, False)):
+        c = np.zeros((Ny, Nx), complex)
+        c[0, col] = 1.0
+        s = _surf(Nx, Ny, dx, dy, c)
+        assert (s.kx_s[16] == np.float32(2 * np.pi / 20) and
+                float(s.kx_s[16]) > 2 * np.pi / 20)
+        h = osurf.synth(s, 0.4, ('hMTF',), dtype=np.float64)['hMTF']
+        assert (np.max(np.abs(h)) > 0) == kept, col
