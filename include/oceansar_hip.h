@@ -214,6 +214,10 @@ typedef struct osar_bragg_desc {
   int64_t diff_stride;      /* elements between snapshot slots of diffx/diffy */
   int64_t amp_stride;       /* elements between amplitude slots */
   double* amp64;            /* out (fp64 = 1): as amp, float64 */
+  const float* hmtf;        /* NULL, or the hydrodynamic MTF field of the slopes'
+                               snapshot (same layout and diff_stride as diffx): every
+                               RCS is multiplied by 1 + max(hmtf, -1), computed in
+                               float32 (use_hmtf, sar_raw_nompi.py:448-456) */
 } osar_bragg_desc;
 
 int osar_facet_bragg(osar_ctx* ctx, const osar_bragg_desc* d, void* stream);

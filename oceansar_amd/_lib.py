@@ -53,7 +53,7 @@ class BraggDesc(ctypes.Structure):
         [(n, c_int32) for n in ('n_refresh', 'snap0')] + \
         [(n, c_void_p) for n in ('refresh_pulse', 'refresh_snap')] + \
         [(n, ctypes.c_int64) for n in ('diff_stride', 'amp_stride')] + \
-        [('amp64', c_void_p)]
+        [('amp64', c_void_p), ('hmtf', c_void_p)]
 
 
 NUM_FIELDS = 15   # OSAR_NUM_FIELDS: output slots of osar_surface_desc

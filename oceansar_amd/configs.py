@@ -96,7 +96,7 @@ plot_rcmc_time=False
 plot_image_valid=False
 
 [srg]
-use_hmtf=False
+use_hmtf={hmtf}
 nesz=-30.
 wh_tol=2.73
 scat_spec_enable={specular}
@@ -131,7 +131,7 @@ def cfg_text(L, dx, n_pulses, K, U, wind_dir=0., num_ch=1, b_ati=None, pol='vv',
              spread='elfouhaily', specular=False, swell=False, seed=1,
              f0=5.405e9, inc=35., prf=1800., rg_bw=100e6, alt=514e3, v=7600.,
              swell_wl=200., Hs=2., swell_dir=0., choppy=False, dt=None, b_xti=None,
-             frozen=False):
+             frozen=False, hmtf=False):
     antL = antenna_length_for(n_pulses, prf, v, L, inc, alt, f0)
     b_ati = [0.] * num_ch if b_ati is None else list(b_ati)[:num_ch]
     freq_r = np.sqrt(const.g * 2 * np.pi / swell_wl) / (2 * np.pi)
@@ -143,7 +143,7 @@ def cfg_text(L, dx, n_pulses, K, U, wind_dir=0., num_ch=1, b_ati=None, pol='vv',
         b_xti=_arr([0.] * num_ch if b_xti is None else list(b_xti)[:num_ch]),
         npulses=n_pulses, L=L, dx=dx, dt=repr(float(dt)), K=K, wdir=wind_dir, U=U,
         swell=swell, freq_r=repr(float(freq_r)), swell_dir=swell_dir, Hs=Hs,
-        spread=spread, choppy=choppy, specular=specular, frozen=frozen)
+        spread=spread, choppy=choppy, specular=specular, frozen=frozen, hmtf=hmtf)
 
 
 WORKLOADS = ('small', 'ati_swell', 'large')

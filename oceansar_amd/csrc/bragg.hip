@@ -144,6 +144,7 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
   const size_t snap = d.n_refresh > 0 ? (size_t)(d.refresh_snap[ri] - d.snap0) : 0;
   const float* diffx_s = d.diffx + snap * d.diff_stride;
   const float* diffy_s = d.diffy + snap * d.diff_stride;
+  const float* hmtf_s = d.hmtf ? d.hmtf + snap * d.diff_stride : nullptr;
   float* amp_s = d.amp + (size_t)ri * d.amp_stride;
   const double az_now = (pulse * d.t_step - d.t_span / 2.) * d.v_ground;
   double sums[2] = {0., 0.};
@@ -269,6 +270,12 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
     const float T_vv = wgt * (F0 * cabs2f(caddf(cscalef(b_vv, b), cscalef(b_hh, a))));
     const float T_hh = wgt * (F0 * cabs2f(caddf(cscalef(b_hh, b), cscalef(b_vv, a))));
     const float base = S1 * inv_kb;
+    // hydrodynamic modulation: 1 + hMTF with hMTF below -1 set to -1
+    float mod = 1.f;
+    if (hmtf_s) {
+      const float h = hmtf_s[idx];
+      mod = 1.f + (h < -1.f ? -1.f : h);
+    }
 #pragma unroll
     for (int comp = 0; comp < 2; ++comp) {
       const float cth = comp == 0 ? cth0 : -cth0;
@@ -286,6 +293,7 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d
         // romeiser97.py:366-392 (DP: hh masks only the plus component)
         const bool mask = bad && !(POL == 2 && is_hh && comp == 0);
         if (mask) rcs = 0.f;
+        rcs = rcs * mod;
         amp_s[((size_t)ip * 2 + comp) * n + idx] = sqrtf(rcs);
         sums[ip] += (double)rcs;
       }
@@ -342,6 +350,7 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_fp64_kernel(osar_bragg_d
   const size_t snap = d.n_refresh > 0 ? (size_t)(d.refresh_snap[ri] - d.snap0) : 0;
   const float* diffx_s = d.diffx + snap * d.diff_stride;
   const float* diffy_s = d.diffy + snap * d.diff_stride;
+  const float* hmtf_s = d.hmtf ? d.hmtf + snap * d.diff_stride : nullptr;
   double* amp_s = d.amp64 + (size_t)ri * d.amp_stride;
   const double az_now = (pulse * d.t_step - d.t_span / 2.) * d.v_ground;
   double sums[2] = {0., 0.};
@@ -444,6 +453,12 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_fp64_kernel(osar_bragg_d
     const double T_vv = wgt * (F0 * cabs2(cadd(cscale(b_vv, b), cscale(b_hh, a))));
     const double T_hh = wgt * (F0 * cabs2(cadd(cscale(b_hh, b), cscale(b_vv, a))));
     const double base = S1 * inv_kb;
+    // hydrodynamic modulation: 1 + hMTF (float32) with hMTF below -1 set to -1
+    double mod = 1.;
+    if (hmtf_s) {
+      const float h = hmtf_s[idx];
+      mod = (double)(1.f + (h < -1.f ? -1.f : h));
+    }
     for (int comp = 0; comp < 2; ++comp) {
       const double cth = comp == 0 ? cth0 : -cth0;
       double g;
@@ -459,6 +474,7 @@ __global__ __launch_bounds__(kThreads) void facet_bragg_fp64_kernel(osar_bragg_d
         // romeiser97.py:366-392 (DP: hh masks only the plus component)
         const bool mask = bad && !(POL == 2 && is_hh && comp == 0);
         if (mask) rcs = 0.;
+        rcs = rcs * mod;
         amp_s[((size_t)ip * 2 + comp) * n + idx] = sqrt(rcs);
         sums[ip] += rcs;
       }

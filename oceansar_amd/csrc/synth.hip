@@ -331,11 +331,10 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
             // M_h(-k) = M_h(k): H(M_h a) = Re(M_h) H(a) + i Im(M_h) G(a)
             const double2 M = hmtf_mult((float)kx, kyj, om);
             ha = make_double2(M.x * h.x - M.y * g.y, M.x * h.y + M.y * g.x);
-            hb = make_double2(0., 0.);
           } else {
             ha = cmul2(mono_c(da, mono_x(q.ca, kx, ik, om) * ya), anti_a ? g : h);
-            hb = cmul2(mono_c(db, mono_x(q.cb, kx, ik, om) * yb), anti_b ? g : h);
           }
+          hb = cmul2(mono_c(db, mono_x(q.cb, kx, ik, om) * yb), anti_b ? g : h);
           if (in) {
             // ha + i hb at k, conj(ha) + i conj(hb) at -k
             S[pad(i)] = make_double2(ha.x - hb.y, ha.y + hb.x);
@@ -572,8 +571,6 @@ int make_plan(const osar_surface_desc* d, QPlan* q_out, int* nq) {
     }
     items.push_back(Item{f, kMult[f]});
   }
-  // M_h is not a monomial: its spectrum takes it alone (component a)
-  if (d->fields & OSAR_F_HMTF) q[n++] = QPlan{M_HMTF, M_NONE, 14, -1, 0, 0, 0};
   // slope_fp32: Diffx and Diffy share one complex64 spectrum, the other fields
   // pair up in complex128 spectra
   std::vector<Item> lo;
@@ -581,6 +578,17 @@ int make_plan(const osar_surface_desc* d, QPlan* q_out, int* nq) {
     std::vector<Item> keep;
     for (const Item& it : items) (it.out == 3 || it.out == 4 ? lo : keep).push_back(it);
     items.swap(keep);
+  }
+  // M_h is not a monomial: it is component a of its spectrum, whose component b
+  // is the odd monomial field out, if any
+  if (d->fields & OSAR_F_HMTF) {
+    QPlan h{M_HMTF, M_NONE, 14, -1, 0, 0, 0};
+    if (items.size() % 2) {
+      h.mb = items.back().m;
+      h.outb = items.back().out;
+      items.pop_back();
+    }
+    q[n++] = h;
   }
   for (size_t i = 0; i < items.size(); i += 2) {
     const bool two = i + 1 < items.size();

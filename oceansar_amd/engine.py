@@ -283,13 +283,15 @@ class DeviceScene:
         _lib.check(self.ctx.lib.osar_fill_zero_f64(self.ctx.h, _lib.ptr(t), t.numel(),
                                                    _lib.stream_ptr()))
 
-    def bragg(self, diffx, diffy, pulse, amp_out, rcs_sum_out=None):
+    def bragg(self, diffx, diffy, pulse, amp_out, rcs_sum_out=None, hmtf=None):
         """sqrt(RCS) of every facet at refresh pulse `pulse` into amp_out [npol][2][Nx][Ny]
-        (float32; float64 runs the FP64 facet model)."""
+        (float32; float64 runs the FP64 facet model). hmtf: the snapshot's hMTF
+        field, which modulates the RCS (use_hmtf)."""
         import torch
         d = self._bragg_desc()
         d.pulse = int(pulse)
         d.diffx, d.diffy = _lib.ptr(diffx), _lib.ptr(diffy)
+        d.hmtf = _lib.ptr(hmtf)
         d.fp64 = 1 if amp_out.dtype == torch.float64 else 0
         if d.fp64:
             d.amp64 = _lib.ptr(amp_out)
@@ -298,11 +300,13 @@ class DeviceScene:
         d.rcs_sum = _lib.ptr(rcs_sum_out)
         _lib.check(self.ctx.lib.osar_facet_bragg(self.ctx.h, ctypes.byref(d), _lib.stream_ptr()))
 
-    def bragg_refreshes(self, r_first, r_last, diffx, diffy, snap0, amp_out, rcs_sum_out):
+    def bragg_refreshes(self, r_first, r_last, diffx, diffy, snap0, amp_out, rcs_sum_out,
+                        hmtf=None):
         """Refreshes r_first..r_last in one launch: slopes from snapshot slots
         (diffx/diffy [slot][Nx][Ny], slot 0 = snapshot snap0), amplitudes into
         amp_out[0 .. r_last - r_first] (float32, or float64 for the FP64 facet
-        model), sums into rcs_sum_out[r_first .. r_last]."""
+        model), sums into rcs_sum_out[r_first .. r_last]. hmtf: the snapshots'
+        hMTF fields (same slots) when the RCS is modulated (use_hmtf)."""
         import torch
         n = r_last - r_first + 1
         d = self._bragg_desc()
@@ -312,6 +316,7 @@ class DeviceScene:
         d.refresh_pulse = self.refresh_pulse.data_ptr() + 4 * r_first
         d.refresh_snap = self.refresh_snap.data_ptr() + 4 * r_first
         d.diffx, d.diffy = _lib.ptr(diffx), _lib.ptr(diffy)
+        d.hmtf = _lib.ptr(hmtf)
         d.diff_stride = diffx[0].numel()
         if d.fp64:
             d.amp64 = _lib.ptr(amp_out)
@@ -432,6 +437,8 @@ class DeviceScene:
         f = ['Dx', 'Dy', 'Dz']
         if self.sc.bragg_enable:
             f += ['Diffx', 'Diffy']
+            if self.sc.use_hmtf:
+                f += ['hMTF']
         if self.sc.spec_enable:
             f += ['Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
         return [n for n in _lib.FIELD_ORDER if n in f]
@@ -568,15 +575,18 @@ class DeviceScene:
             if r_first == r0 and int(sc.rcs_refresh[r0]) // K < m0:
                 # refresh pulse before this chunk: synthesise its slopes alone
                 pr = int(sc.rcs_refresh[r0])
+                names = ['Diffx', 'Diffy'] + (['hMTF'] if sc.use_hmtf else [])
                 tmp = {n: torch.empty((1, sc.Nx, sc.Ny), dtype=torch.float32, device=self.dev)
-                       for n in ('Diffx', 'Diffy')}
-                self.synth(self.snap_times[pr // K:pr // K + 1], tmp, ['Diffx', 'Diffy'],
+                       for n in names}
+                self.synth(self.snap_times[pr // K:pr // K + 1], tmp, names,
                            self.slope_fp32 and not st.fp64)
-                self.bragg(tmp['Diffx'][0], tmp['Diffy'][0], pr, amp[0], st.rcs_sum[r0])
+                self.bragg(tmp['Diffx'][0], tmp['Diffy'][0], pr, amp[0], st.rcs_sum[r0],
+                           tmp['hMTF'][0] if sc.use_hmtf else None)
                 r_first += 1
             if r_first <= r1:
                 self.bragg_refreshes(r_first, r1, buf['Diffx'], buf['Diffy'], m0,
-                                     amp[r_first - r0:], st.rcs_sum)
+                                     amp[r_first - r0:], st.rcs_sum,
+                                     buf['hMTF'] if sc.use_hmtf else None)
             self._mark(timer, 'bragg', ev)
             st.held, st.held_slot = r1, r1 - r0
         st.chunk = (p, q1, m0, r0)

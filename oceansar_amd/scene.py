@@ -84,8 +84,6 @@ class SarScene:
         if self.bragg_enable and self.bragg_model_name != 'romeiser97':
             raise NotImplementedError('RCS model %s for Bragg scattering not implemented'
                                       % self.bragg_model_name)
-        if self.use_hmtf:
-            raise NotImplementedError('use_hmtf is not supported')
 
         # -- [sar] (sar_raw_nompi.py:84-112)
         self.inc_angle = np.deg2rad(sar.inc_angle)

@@ -25,6 +25,8 @@ class SnapshotCache:
         self.keep = int(keep)
         if compute is None:
             compute = ('D', 'Diff', 'Diff2') if sc.spec_enable else ('D', 'Diff')
+            if getattr(sc, 'use_hmtf', False) and sc.bragg_enable:
+                compute = compute + ('hMTF',)
         self.compute = compute
         self.cache = {}
 
@@ -71,14 +73,28 @@ def beam_pattern(sc, sin_az):
     return np.sinc(sc.ant_l_tx / l0 * sin_az) * np.sinc(sc.ant_l_rx / l0 * sin_az)
 
 
+def hmtf_modulation(hmtf):
+    """1 + hMTF after hMTF < -1 is set to -1, in float32 as the reference's
+    float32 field (sar_raw_nompi.py:448-456)."""
+    h = np.array(hmtf, dtype=np.float32)
+    h[np.where(h < -1)] = -1
+    return (1 + h).astype(np.float64)
+
+
 def bragg_rcs(sc, snaps, p):
-    """Bragg RCS per polarisation at refresh pulse p (sar_raw_nompi.py:432-446)."""
+    """Bragg RCS per polarisation at refresh pulse p (sar_raw_nompi.py:432-456),
+    modulated by the hydrodynamic MTF when use_hmtf is set."""
     f = lerp_fields(snaps, sc, p, ('Diffx', 'Diffy'))
     _, _, az_proj = row_geometry(sc, p)
     b = sc.bragg
-    return _bragg.rcs_romeiser97(sc.inc, az_proj[:, 0], f['Diffx'], f['Diffy'], b.k0,
-                                 sc.pol, b.wind_dir, b.d, b.spec_k, b.spec_S, b.spread_id,
-                                 b.wind_U, b.wind_fetch, sc.surface.dx, sc.surface.dy)
+    rcs = _bragg.rcs_romeiser97(sc.inc, az_proj[:, 0], f['Diffx'], f['Diffy'], b.k0,
+                                sc.pol, b.wind_dir, b.d, b.spec_k, b.spec_S, b.spread_id,
+                                b.wind_U, b.wind_fetch, sc.surface.dx, sc.surface.dy)
+    if getattr(sc, 'use_hmtf', False):
+        # the float32 field at the refresh pulse, like the slopes
+        m = hmtf_modulation(lerp_fields(snaps, sc, p, ('hMTF',))['hMTF'])
+        rcs = {pol: v * m for pol, v in rcs.items()}
+    return rcs
 
 
 def facet_range(sr, cos_inc, sin_inc, az, sin_az, Dx, Dy, Dz, sr0):

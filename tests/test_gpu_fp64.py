@@ -60,7 +60,7 @@ def test_fp64_bragg_kernel_matches_oracle(need_gpu, tmp_path_factory):
         assert abs(tot[ip].item() / rcs[pol].sum() - 1) < 1e-12
 
 
-@pytest.mark.parametrize('case', ['plain', 'dp_xti', 'specular'])
+@pytest.mark.parametrize('case', ['plain', 'dp_xti', 'specular', 'hmtf'])
 def test_fp64_path_matches_oracle(need_gpu, tmp_path_factory, case):
     import torch
     import oracle.sar as osar
@@ -72,9 +72,13 @@ def test_fp64_path_matches_oracle(need_gpu, tmp_path_factory, case):
         sc = _scene(tmp_path_factory, 'f2', pol='DP', num_ch=3, b_ati=[0., 7., 15.],
                     b_xti=[0., 30., -12.], spread='elfouhaily', K=4, wind_dir=30.)
         p0, p1 = 500, 530
-    else:
+    elif case == 'specular':
         sc = _scene(tmp_path_factory, 'f3', base='ati_swell', L=128., n_pulses=512, inc=8.)
         p0, p1 = 250, 262
+    else:
+        sc = _scene(tmp_path_factory, 'f5', pol='DP', hmtf=True, L=128., K=8,
+                    spread='elfouhaily')
+        p0, p1 = 400, 430
     ds = DeviceScene(sc)
     raw, nrcs = ds.run(p0, p1, chunk_snaps=2, fp64=True)
     torch.cuda.synchronize()

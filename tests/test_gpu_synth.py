@@ -232,6 +232,35 @@ def test_end_to_end_specular(need_gpu, tmp_path_factory, pol):
     np.testing.assert_allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-6)
 
 
+@pytest.mark.parametrize('pol', ['vv', 'DP'])
+def test_end_to_end_hmtf(need_gpu, tmp_path_factory, pol):
+    """use_hmtf (sar_raw_nompi.py:448-456): the Bragg RCS of every refresh times
+    1 + hMTF of its snapshot (hMTF synthesised beside the slopes, paired with
+    the odd displacement field in one spectrum), on a window crossing chunks and
+    refreshes; the modulation moves the NRCS by more than the parity bound."""
+    import torch
+    import oracle.sar as osar
+    from oceansar_amd.engine import DeviceScene
+    sc = _scene(tmp_path_factory, 'hm' + pol, pol=pol, hmtf=True, L=128., K=8,
+                spread='elfouhaily')
+    assert sc.use_hmtf
+    ds = DeviceScene(sc)
+    assert 'hMTF' in ds.fields_needed()
+    p0, p1 = 400, 470
+    raw, nrcs = ds.run(p0, p1, chunk_snaps=3)
+    torch.cuda.synchronize()
+    got = raw[:, :, p0:p1].cpu().numpy().astype(np.complex128)
+    ref, ref_nrcs = osar.simulate(sc, p0, p1)
+    e, ph = rel_rms(got, ref), phase_rms(got, ref)
+    print('hMTF %s rel RMS %.3e phase RMS %.3e' % (pol, e, ph))
+    assert e < 1e-4 and ph < 1e-3
+    print('NRCS max rel diff %.2e' % np.max(np.abs(nrcs[:, p0:p1] / ref_nrcs - 1)))
+    np.testing.assert_allclose(nrcs[:, p0:p1], ref_nrcs, rtol=1e-6)
+    sc.use_hmtf = False
+    _, plain = DeviceScene(sc).run(p0, p1, chunk_snaps=3)
+    assert np.max(np.abs(plain[:, p0:p1] / nrcs[:, p0:p1] - 1)) > 1e-3
+
+
 def test_overlapped_pipeline_matches_serial(need_gpu, tmp_path_factory):
     """Two-stream pipeline (synthesis/Bragg of chunk c+1 beside the accumulation of
     chunk c, double-buffered) gives the single-stream result, and the oracle's."""
