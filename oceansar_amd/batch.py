@@ -302,13 +302,10 @@ def footprint_bytes(sc, chunk=32):
     """Device bytes of one scene run: the resident tables of DeviceScene, the
     chunk buffers of DeviceScene.start and the raw cube. The synthesis
     workspace belongs to the stream's context (workspace_bytes)."""
+    from .engine import scene_fields
     n2 = sc.Nx * sc.Ny
     npol = len(sc.pols)
-    names = {'Dx', 'Dy', 'Dz'}
-    if sc.bragg_enable:
-        names |= {'Diffx', 'Diffy'}
-    if sc.spec_enable:
-        names |= {'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy'}
+    names = scene_fields(sc)
     tables = 8 * (5 * sc.Nx + sc.Ny) + 8 * n2 + 4 * (sc.Nx + sc.Ny)      # columns, rows, coefs, k
     tables += 8 * (sc.az_steps // sc.K + 2)                            # snapshot times
     if sc.bragg_enable:
@@ -339,11 +336,8 @@ def workspace_bytes(sc, chunk=32):
     snapshot of a chunk one complex128 plane per packed spectrum (and, for
     Bragg-only scenes, one complex64 plane of the two slopes), and the float64
     dispersion table (osar_surface_workspace_bytes)."""
+    from .engine import scene_fields, spectra_bytes
     n2 = sc.Nx * sc.Ny
-    if sc.spec_enable:
-        per = 16 * 4                                   # 8 fields: 4 complex128 spectra
-    elif sc.bragg_enable:
-        per = 16 * 2 + 8                               # (Dz, Dx), (Dy), slopes in complex64
-    else:
-        per = 16 * 2                                   # (Dz, Dx), (Dy)
+    per = spectra_bytes(scene_fields(sc), sc.surface.choppy,
+                        sc.bragg_enable and not sc.spec_enable)
     return int(per * (chunk + 1) * n2 + 8 * n2)

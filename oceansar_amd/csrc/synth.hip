@@ -34,7 +34,7 @@
 namespace {
 
 constexpr double kG = 9.80665;
-constexpr int kMaxQ = 8;
+constexpr int kMaxQ = 10;
 constexpr int kMaxN = 2048;
 
 enum Mult : int {
@@ -49,6 +49,9 @@ enum Mult : int {
   M_NKXKY,   // -kx ky       (Diffxy)
   M_NKX2KI,  // -kx^2 / k    (choppy denominator x)
   M_NKY2KI,  // -ky^2 / k    (choppy denominator y)
+  M_NIKX3KI,   // -i kx^3 / k       (choppy curvature terms, ocean.py:466-468)
+  M_NIKY3KI,   // -i ky^3 / k
+  M_NIKX2KYKI, // -i kx^2 ky / k
   M_WKXN,    // omega kx/k         (Vx: -Im ifft2(kx/k (-i omega) a))
   M_WKYN,    // omega ky/k         (Vy)
   M_NIW,     // -i omega           (Vz)
@@ -62,6 +65,8 @@ struct QPlan {
   int ma, mb;      // multipliers of the real and imaginary component
   int outa, outb;  // output field index (0..7) or -1
   int choppy;      // 1: single output = Re / (1 + Im) into outa
+  int keep;        // 1: the column pass writes the FP64 result back in place
+                   // (terms of the choppy curvatures, combined by synth_choppy_kernel)
   int ca, cb;      // the multipliers as monomial codes (mono_code)
   int f32;         // 1: the row pass stores this spectrum as complex64 (else complex128)
   int cbw, lcbw;   // columns per contiguous run of a workspace row (128 bytes) and log2
@@ -90,6 +95,11 @@ struct SynthArgs {
   long long snap_bytes; // workspace bytes per snapshot
   double* omega;        // [ny][nx] omega(|k|), filled by omega_kernel
   QPlan q[kMaxQ];
+  // choppy curvatures: the keep spectra A = (a_x, a_xx), B = (a_y, a_yy),
+  // C = (Dxx, a_xxx), D = (Dyy, a_yyy), E = (Dxy, a_xxy) (plan indices, -1 = none)
+  // and the outputs Diffx, Diffy, Diffxx, Diffyy, Diffxy (NULL = not requested)
+  int ch_q[5];
+  float* ch_out[5];
 };
 
 // Plan entry at a wave-uniform runtime index. A switch keeps
@@ -104,7 +114,9 @@ __device__ __forceinline__ QPlan plan_at(const SynthArgs& a, int i) {
     case 4: return a.q[4];
     case 5: return a.q[5];
     case 6: return a.q[6];
-    default: return a.q[7];
+    case 7: return a.q[7];
+    case 8: return a.q[8];
+    default: return a.q[9];
   }
 }
 
@@ -117,14 +129,16 @@ __device__ __forceinline__ double mono_y(int code, double ky) {
   if (code & kMonoNone) return 0.;
   const int py = (code >> 2) & 3;
   double y = py >= 1 ? ky : 1.;
-  if (py == 2) y = y * ky;
+  if (py >= 2) y = y * ky;
+  if (py == 3) y = y * ky;
   return (code & kMonoNeg) ? -y : y;
 }
 
 __device__ __forceinline__ double mono_x(int code, double kx, double ik, double om) {
   const int px = code & 3;
   double f = px >= 1 ? kx : 1.;
-  if (px == 2) f = f * kx;
+  if (px >= 2) f = f * kx;
+  if (px == 3) f = f * kx;
   if (code & 16) f = f * ik;
   const int po = (code >> 8) & 3;
   if (po >= 1) f = f * om;
@@ -478,6 +492,16 @@ __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(Synt
   const double scale = 1.0 / ((double)nx * (double)ny);
   const double t = a.times[ti];
   const size_t plane = (size_t)nx * ny;
+  if (q.keep) {
+    // back into this block's own workspace positions (row y, column c), scaled
+    double2* w = (double2*)wp + wofs;
+    for (int e = tid; e < ny * cb; e += kColThreads) {
+      const int c = e >> a.logy, y = e & (ny - 1);
+      const double2 z = C[pad(e)];
+      store_stream(&w[(size_t)y * cbw + c], make_double2(z.x * scale, z.y * scale));
+    }
+    return;
+  }
   float* const outa = q.pa;
   float* const outb = q.pb;
   for (int e = tid; e < ny * cb; e += kColThreads) {
@@ -507,6 +531,46 @@ __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(Synt
     }
   }
 }
+// Choppy curvatures (ocean.py:462-475) from the FP64 keep spectra of the
+// column pass, per grid point and snapshot:
+//   Diffx = a_x / (1 + a_xx), Diffy = a_y / (1 + a_yy),
+//   Diffxx = ((1 + a_xx) Dxx - a_xxx a_x) / (1 + a_xx)^3,
+//   Diffyy = ((1 + a_yy) Dyy - a_yyy a_y) / (1 + a_yy)^3,
+//   Diffxy = ((1 + a_xx) Dxy - a_xxy a_x) / ((1 + a_xx)^2 (1 + a_yy)).
+// Thread = one (x, y) with y fastest, so the float32 stores are contiguous.
+__global__ __launch_bounds__(256) void synth_choppy_kernel(SynthArgs a) {
+  const int64_t plane = (int64_t)a.nx * a.ny;
+  const int64_t n = plane * a.n_times;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int ti = (int)(e / plane);
+    const int64_t r = e - (int64_t)ti * plane;
+    const int x = (int)(r >> a.logy), y = (int)(r & (a.ny - 1));
+    double2 v[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const QPlan q = plan_at(a, a.ch_q[k]);
+      const double2* w = (const double2*)(a.work + (size_t)ti * (size_t)a.snap_bytes +
+                                          (size_t)q.off);
+      v[k] = load_stream(&w[((size_t)(x >> q.lcbw) * a.ny + y) * q.cbw + (x & (q.cbw - 1))]);
+    }
+    const double ax = v[0].x, axx = v[0].y, ay = v[1].x, ayy = v[1].y;
+    const double ux = 1. + axx, uy = 1. + ayy;
+    const size_t o = (size_t)e;
+    if (a.ch_out[0]) __builtin_nontemporal_store((float)(ax / ux), &a.ch_out[0][o]);
+    if (a.ch_out[1]) __builtin_nontemporal_store((float)(ay / uy), &a.ch_out[1][o]);
+    if (a.ch_out[2])
+      __builtin_nontemporal_store((float)((ux * v[2].x - v[2].y * ax) / (ux * ux * ux)),
+                                  &a.ch_out[2][o]);
+    if (a.ch_out[3])
+      __builtin_nontemporal_store((float)((uy * v[3].x - v[3].y * ay) / (uy * uy * uy)),
+                                  &a.ch_out[3][o]);
+    if (a.ch_out[4])
+      __builtin_nontemporal_store((float)((ux * v[4].x - v[4].y * ax) / (ux * ux * uy)),
+                                  &a.ch_out[4][o]);
+  }
+}
+
 int ilog2(int n) {
   int l = 0;
   while ((1 << l) < n) ++l;
@@ -530,6 +594,9 @@ int mono_code(int m) {
     case M_NKXKY: return c(1, 1, 0, 0, 1);
     case M_NKX2KI: return c(2, 0, 1, 0, 1);
     case M_NKY2KI: return c(0, 2, 1, 0, 1);
+    case M_NIKX3KI: return c(3, 0, 1, 1, 1);
+    case M_NIKY3KI: return c(0, 3, 1, 1, 1);
+    case M_NIKX2KYKI: return c(2, 1, 1, 1, 1);
     case M_WKXN: return c(1, 0, 1, 0, 0, 1);
     case M_WKYN: return c(0, 1, 1, 0, 0, 1);
     case M_NIW: return c(0, 0, 0, 1, 1, 1);
@@ -541,10 +608,24 @@ int mono_code(int m) {
   }
 }
 
-int make_plan(const osar_surface_desc* d, QPlan* q_out, int* nq) {
-  // Real outputs in pairing order with their multipliers. At most 8 spectra
-  // result (14 monomial fields pair into 7, M_h takes one; choppy slopes take
-  // one each but exclude the 3 curvatures); q has room for twice that.
+// A spectrum of components a and b (multipliers ma, mb into fields outa, outb).
+QPlan spectrum(int ma, int mb, int outa, int outb) {
+  QPlan p{};
+  p.ma = ma;
+  p.mb = mb;
+  p.outa = outa;
+  p.outb = outb;
+  return p;
+}
+
+// The spectra of a descriptor. ch_q (may be NULL): plan indices of the five
+// keep spectra of the choppy curvatures, -1 when there are none.
+int make_plan(const osar_surface_desc* d, QPlan* q_out, int* nq, int* ch_q = nullptr) {
+  // Real outputs in pairing order with their multipliers. At most 10 spectra
+  // result (14 monomial fields pair into 7 and M_h takes one; choppy slopes
+  // take one each but then no curvatures; choppy curvatures take five, from
+  // which the slopes come too, beside at most 9 monomial fields and M_h); q
+  // has room for more.
   struct Item { int out, m; };
   QPlan q[2 * kMaxQ];
   static const int kMult[kNumFields] = {M_IKXN, M_IKYN, M_ONE, M_IKX, M_IKY, M_NKX2, M_NKY2,
@@ -558,15 +639,26 @@ int make_plan(const osar_surface_desc* d, QPlan* q_out, int* nq) {
     osar_set_error("unknown field bits 0x%x", d->fields & ~((1 << kNumFields) - 1));
     return OSAR_EINVAL;
   }
-  if (d->choppy && (d->fields & (OSAR_F_DIFFXX | OSAR_F_DIFFYY | OSAR_F_DIFFXY))) {
-    osar_set_error("choppy second derivatives are not supported by osar_surface_synth");
-    return OSAR_EUNSUPPORTED;
+  const bool curv = (d->fields & (OSAR_F_DIFFXX | OSAR_F_DIFFYY | OSAR_F_DIFFXY)) != 0;
+  const bool choppy2 = d->choppy && curv;
+  int chq[5] = {-1, -1, -1, -1, -1};
+  if (choppy2) {
+    // ocean.py:462-475: a_x, a_xx | a_y, a_yy | Dxx, a_xxx | Dyy, a_yyy | Dxy, a_xxy
+    static const int kA[5] = {M_IKX, M_IKY, M_NKX2, M_NKY2, M_NKXKY};
+    static const int kB[5] = {M_NKX2KI, M_NKY2KI, M_NIKX3KI, M_NIKY3KI, M_NIKX2KYKI};
+    for (int k = 0; k < 5; ++k) {
+      chq[k] = n;
+      q[n] = spectrum(kA[k], kB[k], -1, -1);
+      q[n++].keep = 1;
+    }
   }
   for (int oi = 0; oi < kNumFields - 1; ++oi) {
     const int f = kOrder[oi];
     if (!(d->fields & (1 << f))) continue;
+    if (choppy2 && f >= 3 && f <= 7) continue;   // from the keep spectra
     if (d->choppy && (f == 3 || f == 4)) {
-      q[n++] = QPlan{f == 3 ? M_IKX : M_IKY, f == 3 ? M_NKX2KI : M_NKY2KI, f, -1, 1, 0, 0};
+      q[n] = spectrum(f == 3 ? M_IKX : M_IKY, f == 3 ? M_NKX2KI : M_NKY2KI, f, -1);
+      q[n++].choppy = 1;
       continue;
     }
     items.push_back(Item{f, kMult[f]});
@@ -582,7 +674,7 @@ int make_plan(const osar_surface_desc* d, QPlan* q_out, int* nq) {
   // M_h is not a monomial: it is component a of its spectrum, whose component b
   // is the odd monomial field out, if any
   if (d->fields & OSAR_F_HMTF) {
-    QPlan h{M_HMTF, M_NONE, 14, -1, 0, 0, 0};
+    QPlan h = spectrum(M_HMTF, M_NONE, 14, -1);
     if (items.size() % 2) {
       h.mb = items.back().m;
       h.outb = items.back().out;
@@ -592,13 +684,13 @@ int make_plan(const osar_surface_desc* d, QPlan* q_out, int* nq) {
   }
   for (size_t i = 0; i < items.size(); i += 2) {
     const bool two = i + 1 < items.size();
-    q[n++] = QPlan{items[i].m, two ? items[i + 1].m : (int)M_NONE, items[i].out,
-                   two ? items[i + 1].out : -1, 0, 0, 0, 0, 0, 0, 0};
+    q[n++] = spectrum(items[i].m, two ? items[i + 1].m : (int)M_NONE, items[i].out,
+                      two ? items[i + 1].out : -1);
   }
   if (!lo.empty()) {
     const bool two = lo.size() > 1;
-    q[n++] = QPlan{lo[0].m, two ? lo[1].m : (int)M_NONE, lo[0].out, two ? lo[1].out : -1, 0,
-                   0, 0, 1, 0, 0, 0};
+    q[n] = spectrum(lo[0].m, two ? lo[1].m : (int)M_NONE, lo[0].out, two ? lo[1].out : -1);
+    q[n++].f32 = 1;
   }
   if (n > kMaxQ) {
     osar_set_error("%d spectra exceed the %d a synthesis call takes", n, kMaxQ);
@@ -610,6 +702,8 @@ int make_plan(const osar_surface_desc* d, QPlan* q_out, int* nq) {
     q_out[i] = q[i];
   }
   *nq = n;
+  if (ch_q)
+    for (int k = 0; k < 5; ++k) ch_q[k] = chq[k];
   return OSAR_OK;
 }
 
@@ -745,8 +839,14 @@ int surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream, unsig
   OSAR_PTR(d->coefs);
   OSAR_PTR(d->times);
   SynthArgs a{};
-  OSAR_REQUIRE(make_plan(d, a.q, &a.nq) == OSAR_OK, OSAR_EUNSUPPORTED, "%s", osar_last_error());
+  OSAR_REQUIRE(make_plan(d, a.q, &a.nq, a.ch_q) == OSAR_OK, OSAR_EUNSUPPORTED, "%s",
+               osar_last_error());
   if (a.nq == 0) return OSAR_OK;
+  {
+    static const int kChOut[5] = {3, 4, 5, 6, 7};   // Diffx, Diffy, Diffxx, Diffyy, Diffxy
+    for (int k = 0; k < 5; ++k)
+      a.ch_out[k] = a.ch_q[0] >= 0 && (d->fields & (1 << kChOut[k])) ? d->out[kChOut[k]] : nullptr;
+  }
   for (int f = 0; f < kNumFields; ++f) {
     if (d->fields & (1 << f)) {
       OSAR_REQUIRE(d->out[f] != nullptr, OSAR_EINVAL, "output field %d requested but NULL", f);
@@ -831,6 +931,12 @@ int surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream, unsig
   else
     hipLaunchKernelGGL(synth_cols_kernel<false>, grid_cols, dim3(kColThreads), lds_cols, s, a);
   OSAR_LAUNCHED("synth_cols_kernel");
+  if (a.ch_q[0] >= 0) {
+    const int64_t n = (int64_t)d->nx * d->ny * d->n_times;
+    const int blocks = (int)((n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192);
+    hipLaunchKernelGGL(synth_choppy_kernel, dim3(blocks), dim3(256), 0, s, a);
+    OSAR_LAUNCHED("synth_choppy_kernel");
+  }
   return OSAR_OK;
 }
 

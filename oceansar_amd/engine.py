@@ -84,6 +84,46 @@ def synth_fields(ctx, surf, kx, ky, coefs, times, out, fields=None, mirror=None,
     return times_t
 
 
+def scene_fields(sc):
+    """The surface fields the pulse loop of scene `sc` reads, in field order."""
+    f = ['Dx', 'Dy', 'Dz']
+    if sc.bragg_enable:
+        f += ['Diffx', 'Diffy']
+        if getattr(sc, 'use_hmtf', False):
+            f += ['hMTF']
+    if sc.spec_enable:
+        f += ['Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
+    return [n for n in _lib.FIELD_ORDER if n in f]
+
+
+def spectra_bytes(fields, choppy, slope_fp32):
+    """Workspace bytes per grid point and snapshot of osar_surface_synth for
+    these fields: 16 per complex128 spectrum, 8 per complex64 one (the plan of
+    osar_surface_synth: monomial fields in pairs, choppy slopes alone, five
+    spectra for choppy curvatures, M_h taking an odd field out)."""
+    fields = set(fields)
+    curv = bool(fields & {'Diffxx', 'Diffyy', 'Diffxy'})
+    choppy2 = choppy and curv
+    n128 = 5 if choppy2 else 0
+    items = [n for n in _lib.FIELD_ORDER[:14] if n in fields]
+    if choppy2:
+        items = [n for n in items if n not in ('Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy')]
+    elif choppy:
+        n128 += sum(1 for n in ('Diffx', 'Diffy') if n in items)
+        items = [n for n in items if n not in ('Diffx', 'Diffy')]
+    n64 = 0
+    if slope_fp32 and not choppy:
+        lo = [n for n in items if n in ('Diffx', 'Diffy')]
+        items = [n for n in items if n not in ('Diffx', 'Diffy')]
+        n64 = 1 if lo else 0
+    m = len(items)
+    if 'hMTF' in fields:
+        n128 += 1
+        m -= m % 2
+    n128 += (m + 1) // 2
+    return 16 * n128 + 8 * n64
+
+
 def arange_grid(k):
     """(k0, dk, 1) when k[i] == k0 + i * dk exactly for every i, with the product
     rounded before the sum (numpy.arange fills its output that way), so the Bragg
@@ -434,14 +474,7 @@ class DeviceScene:
 
     # ------------------------------------------------------------------
     def fields_needed(self):
-        f = ['Dx', 'Dy', 'Dz']
-        if self.sc.bragg_enable:
-            f += ['Diffx', 'Diffy']
-            if self.sc.use_hmtf:
-                f += ['hMTF']
-        if self.sc.spec_enable:
-            f += ['Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy']
-        return [n for n in _lib.FIELD_ORDER if n in f]
+        return scene_fields(self.sc)
 
     def synth(self, times, out, fields=None, slope_fp32=False):
         """Surface fields at `times` into out[name] = float32 [n_times][nx][ny] (x-major)."""

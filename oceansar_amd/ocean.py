@@ -9,9 +9,9 @@ attribute names (Dx, Dy, Dz, Diffx, Diffy, Diffxx, Diffyy, Diffxy, Vx, Vy, Vz,
 Ax, Ay, Az, hMTF) and as device tensors in x-major layout in
 ``surface.device_fields``.
 
-Supported compute flags: 'D', 'Diff', 'Diff2', 'V', 'A', 'hMTF' (choppy
-'Diff2' is not); every requested field of one time comes from one
-osar_surface_synth call, two fields per FP64 inverse FFT.
+Supported compute flags: 'D', 'Diff', 'Diff2', 'V', 'A', 'hMTF', choppy or
+not; every requested field of one time comes from one osar_surface_synth
+call, two fields per FP64 inverse FFT.
 """
 import numpy as np
 
@@ -64,8 +64,6 @@ class OceanSurface:
         for c in self.compute:
             if c not in _COMPUTE:
                 raise NotImplementedError("compute '%s' is not available on the GPU path" % c)
-        if s.choppy and 'Diff2' in self.compute:
-            raise NotImplementedError('choppy second derivatives are not supported')
         self._dev = None
         self.initialized = True
 

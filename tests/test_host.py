@@ -177,8 +177,11 @@ def test_workspace_bytes_match_the_library(tmp_path):
     import ctypes
     from oceansar_amd import _lib, batch, configs
     from oceansar_amd.engine import DeviceScene
+    from oceansar_amd.engine import spectra_bytes
     lib = _lib.load()
-    for kw in (dict(), dict(specular=True, base='ati_swell', L=64.)):
+    for kw in (dict(), dict(specular=True, base='ati_swell', L=64.), dict(hmtf=True),
+               dict(specular=True, choppy=True, base='ati_swell', L=64.),
+               dict(choppy=True, hmtf=True)):
         base = kw.pop('base', 'small')
         sc = configs.load_scene(configs.workload(base, L=kw.pop('L', 128.), **kw), tmp_path)
         names = DeviceScene.fields_needed(types_ns(sc))
@@ -186,9 +189,24 @@ def test_workspace_bytes_match_the_library(tmp_path):
             d = _lib.SurfaceDesc()
             d.nx, d.ny, d.n_times = sc.Nx, sc.Ny, chunk + 1
             d.fields = sum(_lib.FIELD_BITS[n] for n in names)
+            d.choppy = 1 if sc.surface.choppy else 0
             d.slope_fp32 = 1 if (sc.bragg_enable and not sc.spec_enable) else 0
             assert lib.osar_surface_workspace_bytes(ctypes.byref(d)) == \
-                batch.workspace_bytes(sc, chunk), (sc.spec_enable, chunk)
+                batch.workspace_bytes(sc, chunk), (kw, chunk)
+    # every field combination's plan, against the library's
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        names = [n for n in _lib.FIELD_ORDER if rng.random() < 0.4]
+        if not names:
+            continue
+        choppy, f32 = bool(rng.random() < 0.5), bool(rng.random() < 0.5)
+        d = _lib.SurfaceDesc()
+        d.nx, d.ny, d.n_times = 64, 32, 1
+        d.fields = sum(_lib.FIELD_BITS[n] for n in names)
+        d.choppy, d.slope_fp32 = int(choppy), int(f32)
+        want = lib.osar_surface_workspace_bytes(ctypes.byref(d))
+        assert want == spectra_bytes(names, choppy, f32) * 64 * 32 + 8 * 64 * 32, \
+            (names, choppy, f32)
 
 
 def types_ns(sc):
