@@ -60,7 +60,7 @@ int osar_ctx_destroy(osar_ctx* ctx);
 const char* osar_last_error(void);
 /* Library version: major*10000 + minor*100 + patch. */
 int osar_version(void);
-/* sizeof of the descriptor structs (0 echo, 1 bragg, 2 surface, 3 focus), so
+/* sizeof of the descriptor structs (0 echo, 1 bragg, 2 surface, 3 focus, 4 init), so
    that bindings can check their layout; -1 for an unknown id. */
 int64_t osar_struct_size(int which);
 
@@ -287,6 +287,51 @@ int osar_surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream);
    (serialise them on one stream). */
 #define OSAR_SYNTH_SCRATCH_BYTES (256ll << 20)
 int osar_surface_synth_noalloc(osar_ctx* ctx, const osar_surface_desc* d, void* stream);
+
+/* ---- surface initialisation on the device ---------------------------- */
+/* OceanSurface.init's spectrum and coefficients (oceansar/surfaces/ocean.py:
+   131-216) on the device, so that a served job's surface never leaves HBM.
+   Per grid point (row j, column i; row-major [ny][nx]): |k|, the good-k mask
+   |k| > min(kx_res, ky_res) / 2 and 1/|k| in float32 as the reference's
+   tables; theta = angle(exp(i (atan2(ky, kx) - wind_dir_eff))); the Elfouhaily
+   spectrum S(k) (oceansar/spec/elfouhaily.py:43-99) and the spreading G(k,
+   theta) (oceansar/spread/elfouhaily.py:23-56 or cos2s.py:13-18) in FP64,
+   each rounded to float32; dirspec = kinv S G and coefs = nx ny sqrt(2 dirspec
+   kx_res ky_res) cg in float32 as the reference's expressions, plus the
+   directional swell term (oceansar/swell_spec/dir_swell_spec.py:24-39,
+   ocean.py:206-216) when swell is set. The random draws replace the
+   reference's unseeded numpy RNG with a counter-based generator:
+   Philox4x32-10 (Salmon et al., SC'11) with key (seed & 0xffffffff, seed >>
+   32) and counter (e & 0xffffffff, e >> 32, 0, 0) at point e = j nx + i gives
+   x0..x3; cg = sqrt(-2 ln u0) exp(2 pi i u1) / sqrt(2) with u = (x + 0.5)
+   2^-32 from (x0, x1) for the wind sea and (x2, x3) for the swell (FP64,
+   rounded to complex64). Scalars that depend only on the wind and the swell
+   parameters are the caller's, computed as the reference computes them. */
+typedef struct osar_init_desc {
+  int32_t nx, ny;           /* grid, 1 .. 2^30 points */
+  int32_t spread;           /* 0 Elfouhaily, 1 cos^2 */
+  int32_t swell;            /* 1: add the directional swell coefficients */
+  uint64_t seed;            /* Philox key */
+  float kx_res, ky_res;     /* float32 grid steps */
+  float wind_dir_eff;       /* float32 effective wind direction [rad] */
+  float reserved;
+  /* Elfouhaily spectrum: k_p, c_p, Omega = U / c_p, alpha_p, sigma, gamma,
+     alpha_m, c_m, k_m (elfouhaily.py:43-99) */
+  double k_p, c_p, omega_w, alpha_p, sigma, gamma, alpha_m, c_m, k_m;
+  /* Elfouhaily spreading: a_0, a_p, a_m, cK_p, cK_m (spread/elfouhaily.py:26-52) */
+  double a_0, a_p, a_m, cK_p, cK_m;
+  /* directional swell: direction [rad], freq_r [Hz], sigf [Hz], sigs [rad],
+     amplitude (Hs / 4)^2 / sigf / sqrt(2 pi) */
+  double swell_dir, freq_r, sigf, sigs, swell_amp;
+  const float* kx;          /* [nx] float32 wave numbers */
+  const float* ky;          /* [ny] */
+  osar_c32* coefs;          /* out: [ny][nx] complex64 wave coefficients */
+  float* dirspec;           /* out (NULL: not written): [ny][nx] wind-sea kinv S G */
+  double* rmss_sums;        /* out (NULL: not written): [2] sums of dirspec kx^2
+                               kx_res ky_res and dirspec ky^2 kx_res ky_res */
+} osar_init_desc;
+
+int osar_surface_init(osar_ctx* ctx, const osar_init_desc* d, void* stream);
 
 /* ---- SAR focusing ----------------------------------------------------- */
 /* One channel of sar_focus (oceansar/radarsim/sar_processor.py:131-225):

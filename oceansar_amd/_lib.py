@@ -67,6 +67,16 @@ class SurfaceDesc(ctypes.Structure):
         [('out', c_void_p * NUM_FIELDS)]
 
 
+class InitDesc(ctypes.Structure):
+    _fields_ = [(n, c_int32) for n in ('nx', 'ny', 'spread', 'swell')] + \
+        [('seed', ctypes.c_uint64)] + \
+        [(n, ctypes.c_float) for n in ('kx_res', 'ky_res', 'wind_dir_eff', 'reserved')] + \
+        [(n, c_double) for n in ('k_p', 'c_p', 'omega_w', 'alpha_p', 'sigma', 'gamma',
+                                 'alpha_m', 'c_m', 'k_m', 'a_0', 'a_p', 'a_m', 'cK_p', 'cK_m',
+                                 'swell_dir', 'freq_r', 'sigf', 'sigs', 'swell_amp')] + \
+        [(n, c_void_p) for n in ('kx', 'ky', 'coefs', 'dirspec', 'rmss_sums')]
+
+
 class FocusDesc(ctypes.Structure):
     _fields_ = [(n, c_int32) for n in ('n_pol', 'n_az0', 'n_rg0', 'n_az', 'n_rg', 'az_lo',
                                        'az_hi', 'reserved')] + \
@@ -83,7 +93,7 @@ EXPORTS = ['osar_ctx_create', 'osar_ctx_destroy', 'osar_last_error', 'osar_versi
            'osar_fill_zero_c32',
            'osar_echo_deposit', 'osar_facet_bragg', 'osar_fill_zero_f64',
            'osar_surface_workspace_bytes', 'osar_surface_synth', 'osar_surface_synth_noalloc',
-           'osar_sar_focus']
+           'osar_sar_focus', 'osar_surface_init']
 SYNTH_SCRATCH_BYTES = 256 << 20    # OSAR_SYNTH_SCRATCH_BYTES
 
 _lib = None
@@ -116,7 +126,9 @@ def load():
     lib.osar_surface_synth.argtypes = [c_void_p, POINTER(SurfaceDesc), c_void_p]
     lib.osar_surface_synth_noalloc.argtypes = [c_void_p, POINTER(SurfaceDesc), c_void_p]
     lib.osar_sar_focus.argtypes = [c_void_p, POINTER(FocusDesc), c_void_p]
-    for name, cls in ((0, EchoDesc), (1, BraggDesc), (2, SurfaceDesc), (3, FocusDesc)):
+    lib.osar_surface_init.argtypes = [c_void_p, POINTER(InitDesc), c_void_p]
+    for name, cls in ((0, EchoDesc), (1, BraggDesc), (2, SurfaceDesc), (3, FocusDesc),
+                      (4, InitDesc)):
         n = lib.osar_struct_size(name)
         if n != ctypes.sizeof(cls):
             raise ImportError('descriptor %s: C size %d != ctypes size %d'

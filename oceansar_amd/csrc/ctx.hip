@@ -41,6 +41,7 @@ extern "C" int64_t osar_struct_size(int which) {
     case 1: return (int64_t)sizeof(osar_bragg_desc);
     case 2: return (int64_t)sizeof(osar_surface_desc);
     case 3: return (int64_t)sizeof(osar_focus_desc);
+    case 4: return (int64_t)sizeof(osar_init_desc);
     default: return -1;
   }
 }

@@ -18,6 +18,8 @@ POL_CODE = {'vv': 0, 'hh': 1, 'DP': 2}
 
 def _t(a, dtype, device):
     import torch
+    if torch.is_tensor(a):   # already on the device (osar_surface_init coefficients)
+        return a.to(device=device, dtype=dtype).contiguous()
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=device)
 
 

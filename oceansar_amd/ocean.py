@@ -16,7 +16,7 @@ call, two fields per FP64 inverse FFT.
 import numpy as np
 
 from . import raw_io
-from .surface_init import SurfaceInit, init_surface
+from .surface_init import SurfaceInit, init_surface, init_surface_device
 
 _COMPUTE = {'D': ['Dx', 'Dy', 'Dz'], 'Diff': ['Diffx', 'Diffy'],
             'Diff2': ['Diffxx', 'Diffyy', 'Diffxy'], 'V': ['Vx', 'Vy', 'Vz'],
@@ -36,12 +36,25 @@ class OceanSurface:
              wind_U, current_mag, current_dir, dir_swell_dir=0, freq_r=100, sigf=1, sigs=1,
              Hs=2, swell_dir_enable=False, swell_enable=False, swell_ampl=0., swell_dir=0.,
              swell_wl=0., compute=[], opt_res=True, fft_max_prime=2, choppy_enable=False,
-             dirspectrum_func=None, depth=None, dir_swell_spec=None, seed=None):
+             dirspectrum_func=None, depth=None, dir_swell_spec=None, seed=None, on_device=False):
         """ocean.py:29-242. `seed` (extension) seeds the coefficient draws; None draws
-        unseeded, like the reference's global numpy RNG."""
+        unseeded, like the reference's global numpy RNG. on_device (extension):
+        build the spectrum and the coefficients in HBM (osar_surface_init, Philox
+        draws keyed by the seed); wave_coefs is then a device tensor."""
         if dirspectrum_func is not None:
             raise NotImplementedError('external directional spectra (buoy data) are not supported')
         ss = np.random.SeedSequence(seed).spawn(2)
+        if on_device:
+            s = init_surface_device(Lx, Ly, dx, dy, spec_model, spread_model, wind_dir,
+                                    wind_fetch, wind_U, current_mag, current_dir,
+                                    seed=int(ss[0].generate_state(1, np.uint64)[0]),
+                                    dir_swell_dir=dir_swell_dir, freq_r=freq_r, sigf=sigf,
+                                    sigs=sigs, Hs=Hs, swell_dir_enable=swell_dir_enable,
+                                    swell_enable=swell_enable, opt_res=opt_res,
+                                    fft_max_prime=fft_max_prime, choppy_enable=choppy_enable,
+                                    depth=depth, cutoff_wl=cutoff_wl, device=self.device)
+            self._adopt(s, compute)
+            return
         rngs = [np.random.Generator(np.random.PCG64(s)) for s in ss]
         s = init_surface(Lx, Ly, dx, dy, spec_model, spread_model, wind_dir, wind_fetch, wind_U,
                          current_mag, current_dir, rng_wave=rngs[0], rng_swell=rngs[1],
@@ -90,7 +103,10 @@ class OceanSurface:
             st.set('choppy_enable', 1 if s.choppy else 0)
             st.set('kx', self.kx)
             st.set('ky', self.ky)
-            st.set('wave_coefs*', s.wave_coefs)
+            coefs = s.wave_coefs
+            if hasattr(coefs, 'cpu'):   # built on the device
+                coefs = coefs.cpu().numpy()
+            st.set('wave_coefs*', coefs)
             st._f.depth = -1.0 if s.depth is None else float(s.depth)
 
     def load(self, state_file, compute=[]):
@@ -138,6 +154,7 @@ class OceanSurface:
             self._dev = (ctx,
                          torch.as_tensor(s.kx_s, device=dev),
                          torch.as_tensor(s.ky_s, device=dev),
+                         s.wave_coefs.to(dev) if torch.is_tensor(s.wave_coefs) else
                          torch.as_tensor(np.ascontiguousarray(s.wave_coefs), device=dev))
         return self._dev
 

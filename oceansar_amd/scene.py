@@ -36,7 +36,7 @@ from .antenna import sinc_bp
 from .closure import RandomScatterers, grid_coherence
 from .range_profile import N_SINC_SAMPLES, SINC_OVS, calc_sinc_vec
 from .spread import SPREAD_IDS
-from .surface_init import init_surface
+from .surface_init import init_surface, init_surface_device
 from .utils import optimize_fftsize
 
 
@@ -60,7 +60,7 @@ class BraggModel:
 class SarScene:
     """All host-side inputs of one raw-data simulation."""
 
-    def __init__(self, cfg, seed=None, n_pulses=None, surface=None):
+    def __init__(self, cfg, seed=None, n_pulses=None, surface=None, device_init=None):
         sar, ocean, srg = cfg.sar, cfg.ocean, cfg.srg
         sim = getattr(cfg, 'sim', None)
         if seed is None:
@@ -104,6 +104,23 @@ class SarScene:
             depth = None if depth.lower() == 'none' else float(depth)
         swell_dir_enable = bool(_get(ocean, 'swell_dir_enable', False))
         swell_enable = bool(_get(ocean, 'swell_enable', False))
+        if device_init is not None and surface is None:
+            # spectrum and coefficients built in HBM (osar_surface_init), keyed by
+            # the wind-sea stream of the scene seed; device_init is the device index
+            surface = init_surface_device(
+                float(ocean.Lx), float(ocean.Ly), float(ocean.dx), float(ocean.dy),
+                str(ocean.spec_model), str(ocean.spread_model),
+                np.deg2rad(ocean.wind_dir), float(ocean.wind_fetch), float(ocean.wind_U),
+                float(ocean.current_mag), np.deg2rad(ocean.current_dir),
+                seed=int(streams[0].generate_state(1, np.uint64)[0]),
+                dir_swell_dir=_get(ocean, 'dir_swell_dir', 0),
+                freq_r=_get(ocean, 'freq_r', 100), sigf=_get(ocean, 'sigf', 1),
+                sigs=_get(ocean, 'sigs', 1), Hs=_get(ocean, 'Hs', 2),
+                swell_dir_enable=swell_dir_enable, swell_enable=swell_enable,
+                opt_res=bool(_get(ocean, 'opt_res', True)),
+                fft_max_prime=int(_get(ocean, 'fft_max_prime', 2)),
+                choppy_enable=bool(_get(ocean, 'choppy_enable', False)),
+                depth=depth, cutoff_wl=_get(ocean, 'cutoff_wl', 'auto'), device=int(device_init))
         self.surface = surface if surface is not None else init_surface(
             float(ocean.Lx), float(ocean.Ly), float(ocean.dx), float(ocean.dy),
             str(ocean.spec_model), str(ocean.spread_model),

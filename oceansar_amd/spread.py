@@ -15,22 +15,27 @@ _S = 0.072
 _X_0 = 22e3
 
 
-def elfouhaily_delta(k, U_10, fetch):
-    """Delta(k) of the Elfouhaily spreading, eq. 57 (elfouhaily.py:26-52)."""
+def elfouhaily_delta_scalars(U_10, fetch):
+    """The wave-number independent terms of Delta(k) (elfouhaily.py:26-52):
+    a_0, a_p, a_m, cK_p, cK_m."""
     k_0 = g / U_10 ** 2
     X = k_0 * fetch
     Omega_c = 0.84 * np.tanh((X / _X_0) ** 0.4) ** (-0.75)
-    cK = np.sqrt(g / k + _S / _rho * k)
     k_p = k_0 * Omega_c ** 2
     cK_p = np.sqrt(g / k_p + _S / _rho * k_p)
     k_m = np.sqrt(_rho * g / _S)
     cK_m = np.sqrt(g / k_m + _S / _rho * k_m)
     C_10 = (0.8 + 0.065 * U_10) * 1e-3
     ustar = np.sqrt(C_10) * U_10
-    a_0 = np.log(2.) / 4.
-    a_p = 4.
-    a_m = 0.13 * ustar / cK_m
-    return np.tanh(a_0 + a_p * (cK / cK_p) ** 2.5 + a_m * (cK_m / cK) ** 2.5)
+    return dict(a_0=np.log(2.) / 4., a_p=4., a_m=0.13 * ustar / cK_m, cK_p=cK_p, cK_m=cK_m)
+
+
+def elfouhaily_delta(k, U_10, fetch):
+    """Delta(k) of the Elfouhaily spreading, eq. 57 (elfouhaily.py:26-52)."""
+    s = elfouhaily_delta_scalars(U_10, fetch)
+    cK = np.sqrt(g / k + _S / _rho * k)
+    return np.tanh(s['a_0'] + s['a_p'] * (cK / s['cK_p']) ** 2.5 +
+                   s['a_m'] * (s['cK_m'] / cK) ** 2.5)
 
 
 def elfouhaily(k, theta, U_10, fetch):

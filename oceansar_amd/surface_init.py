@@ -63,13 +63,9 @@ def _complex_gaussian(rng, shape):
     return (1. / np.sqrt(2) * (re + 1j * im)).astype(np.complex64)
 
 
-def init_surface(Lx, Ly, dx, dy, spec_model, spread_model, wind_dir, wind_fetch,
-                 wind_U, current_mag, current_dir, rng_wave, rng_swell=None,
-                 dir_swell_dir=0, freq_r=100, sigf=1, sigs=1, Hs=2,
-                 swell_dir_enable=False, swell_enable=False, swell_ampl=0.,
-                 swell_dir=0., swell_wl=0., opt_res=True, fft_max_prime=2,
-                 choppy_enable=False, depth=None, cutoff_wl='auto'):
-    """Build the surface grid and complex wave coefficients (ocean.py:100-242)."""
+def _grid(Lx, Ly, dx, dy, wind_dir, wind_U, current_mag, current_dir, opt_res, fft_max_prime,
+          cutoff_wl):
+    """Sizes, grids, current and effective wind of ocean.py:100-137."""
     if cutoff_wl and cutoff_wl != 'auto':
         # ocean.py:185-198 slices with float indices and fails for any value
         raise NotImplementedError("only cutoff_wl='auto' is supported")
@@ -82,19 +78,33 @@ def init_surface(Lx, Ly, dx, dy, spec_model, spread_model, wind_dir, wind_fetch,
         dy = np.float32(Ly / float(Ny))
     x = np.linspace(-Lx / 2., Lx / 2., Nx, dtype=np.float32)
     y = np.linspace(-Ly / 2., Ly / 2., Ny, dtype=np.float32)
-
     current = (current_mag * np.array([np.cos(current_dir),
                                        np.sin(current_dir)])).astype(np.float32)
     U_eff_vec = (wind_U * np.array([np.cos(wind_dir), np.sin(wind_dir)]) -
                  current).astype(np.float32)
-    wind_U_eff = linalg.norm(U_eff_vec)
-    wind_dir_eff = np.arctan2(U_eff_vec[1], U_eff_vec[0])
-
     kx_s = (2 * np.pi * np.fft.fftfreq(Nx, dx)).astype(np.float32)
     ky_s = (2 * np.pi * np.fft.fftfreq(Ny, dy)).astype(np.float32)
+    return dict(Nx=Nx, Ny=Ny, dx=dx, dy=dy, x=x, y=y, current=current,
+                wind_U_eff=linalg.norm(U_eff_vec),
+                wind_dir_eff=np.arctan2(U_eff_vec[1], U_eff_vec[0]),
+                kx_s=kx_s, ky_s=ky_s,
+                kx_res=kx_s[1] - kx_s[0] if Nx > 1 else np.float32(0),
+                ky_res=ky_s[1] - ky_s[0] if Ny > 1 else np.float32(0))
+
+
+def init_surface(Lx, Ly, dx, dy, spec_model, spread_model, wind_dir, wind_fetch,
+                 wind_U, current_mag, current_dir, rng_wave, rng_swell=None,
+                 dir_swell_dir=0, freq_r=100, sigf=1, sigs=1, Hs=2,
+                 swell_dir_enable=False, swell_enable=False, swell_ampl=0.,
+                 swell_dir=0., swell_wl=0., opt_res=True, fft_max_prime=2,
+                 choppy_enable=False, depth=None, cutoff_wl='auto'):
+    """Build the surface grid and complex wave coefficients (ocean.py:100-242)."""
+    g = _grid(Lx, Ly, dx, dy, wind_dir, wind_U, current_mag, current_dir, opt_res,
+              fft_max_prime, cutoff_wl)
+    Nx, Ny, dx, dy, x, y = g['Nx'], g['Ny'], g['dx'], g['dy'], g['x'], g['y']
+    current, wind_U_eff, wind_dir_eff = g['current'], g['wind_U_eff'], g['wind_dir_eff']
+    kx_s, ky_s, kx_res, ky_res = g['kx_s'], g['ky_s'], g['kx_res'], g['ky_res']
     kx, ky = np.meshgrid(kx_s, ky_s)
-    kx_res = kx[0, 1] - kx[0, 0]
-    ky_res = ky[1, 0] - ky[0, 0]
     k = np.sqrt(kx ** 2 + ky ** 2)
     good_k = np.where(k > np.min(np.array([kx_res, ky_res])) / 2.0)
     kinv = np.zeros(k.shape, dtype=np.float32)
@@ -155,6 +165,68 @@ def init_surface(Lx, Ly, dx, dy, spec_model, spread_model, wind_dir, wind_fetch,
                                                      np.tanh(s.swell_k * depth))))
         s.swell_ph0 = float(np.float32(rng_swell.uniform(0., 2. * np.pi)))
     return s
+
+
+def init_surface_device(Lx, Ly, dx, dy, spec_model, spread_model, wind_dir, wind_fetch,
+                        wind_U, current_mag, current_dir, seed, dir_swell_dir=0, freq_r=100,
+                        sigf=1, sigs=1, Hs=2, swell_dir_enable=False, swell_enable=False,
+                        opt_res=True, fft_max_prime=2, choppy_enable=False, depth=None,
+                        cutoff_wl='auto', device=0, ctx=None):
+    """init_surface with the spectrum and the coefficients built on the device
+    (osar_surface_init): wave_coefs and wave_dirspec of the result are device
+    tensors ([Ny, Nx] complex64 and float32) that never pass through the host.
+    The draws come from Philox4x32-10 keyed by `seed` (an integer below 2^64),
+    so the coefficients are not those init_surface draws with numpy Generators.
+    """
+    import ctypes
+    import torch
+    from . import _lib
+    if swell_enable:
+        raise NotImplementedError('monochromatic swell is not supported on the GPU path')
+    if spec_model not in spec_models.models:
+        raise NotImplementedError('%s spectrum function not implemented' % spec_model)
+    if spread_model not in spread_models.SPREAD_IDS:
+        raise NotImplementedError('%s spreading function not implemented' % spread_model)
+    g = _grid(Lx, Ly, dx, dy, wind_dir, wind_U, current_mag, current_dir, opt_res,
+              fft_max_prime, cutoff_wl)
+    Nx, Ny = g['Nx'], g['Ny']
+    ctx = ctx if ctx is not None else _lib.context(device)
+    dev = torch.device('cuda', ctx.device)
+    kx = torch.as_tensor(g['kx_s'], device=dev)
+    ky = torch.as_tensor(g['ky_s'], device=dev)
+    coefs = torch.empty((Ny, Nx), dtype=torch.complex64, device=dev)
+    dirspec = torch.empty((Ny, Nx), dtype=torch.float32, device=dev)
+    sums = torch.empty(2, dtype=torch.float64, device=dev)
+    d = _lib.InitDesc()
+    d.nx, d.ny = Nx, Ny
+    d.spread = spread_models.SPREAD_IDS[spread_model]
+    d.swell = 1 if swell_dir_enable else 0
+    d.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    d.kx_res, d.ky_res = float(g['kx_res']), float(g['ky_res'])
+    d.wind_dir_eff = float(g['wind_dir_eff'])
+    e = spec_models.elfouhaily_scalars(g['wind_U_eff'], wind_fetch)
+    d.k_p, d.c_p, d.omega_w = float(e['k_p']), float(e['c_p']), float(e['Omega'])
+    d.alpha_p, d.sigma, d.gamma = float(e['alpha_p']), float(e['sigma']), float(e['gamma'])
+    d.alpha_m, d.c_m, d.k_m = float(e['alpha_m']), float(e['c_m']), float(e['k_m'])
+    # the host spreading evaluates Delta(k) in FP64 from float(U_eff)
+    sp = spread_models.elfouhaily_delta_scalars(float(g['wind_U_eff']), float(wind_fetch))
+    d.a_0, d.a_p, d.a_m = float(sp['a_0']), float(sp['a_p']), float(sp['a_m'])
+    d.cK_p, d.cK_m = float(sp['cK_p']), float(sp['cK_m'])
+    d.swell_dir, d.freq_r = float(np.radians(dir_swell_dir)), float(freq_r)
+    d.sigf, d.sigs = float(sigf), float(np.radians(sigs))
+    d.swell_amp = float((Hs / 4) ** 2 / sigf / np.sqrt(2 * np.pi))
+    d.kx, d.ky = kx.data_ptr(), ky.data_ptr()
+    d.coefs, d.dirspec, d.rmss_sums = coefs.data_ptr(), dirspec.data_ptr(), sums.data_ptr()
+    _lib.check(ctx.lib.osar_surface_init(ctx.h, ctypes.byref(d), _lib.stream_ptr()))
+    sx, sy = sums.cpu().numpy()      # waits for the launch
+    return SurfaceInit(Lx=Lx, Ly=Ly, Nx=Nx, Ny=Ny, dx=g['dx'], dy=g['dy'], x=g['x'], y=g['y'],
+                       kx_s=g['kx_s'], ky_s=g['ky_s'], kx_res=g['kx_res'], ky_res=g['ky_res'],
+                       wave_coefs=coefs, wave_dirspec=dirspec,
+                       rmss_x=float(np.sqrt(sx)), rmss_y=float(np.sqrt(sy)), wind_U=wind_U,
+                       wind_dir=wind_dir, wind_U_eff=float(g['wind_U_eff']),
+                       wind_dir_eff=float(g['wind_dir_eff']), wind_fetch=wind_fetch,
+                       current=g['current'], depth=depth, choppy=bool(choppy_enable),
+                       extra={'seed': int(seed)})
 
 
 def synthesis_tables(s):
