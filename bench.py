@@ -142,11 +142,14 @@ def cpu_baseline(sc, raw_gpu, sample_pulses, sample_pulses_1t):
     return cpu, acc
 
 
-def _scenes(kws, tag):
+def _scenes(kws, tag, device_init=None):
+    """Scenes of the given cfg keywords; device_init (a device index): their
+    spectra and coefficients are built in HBM (osar_surface_init)."""
     from oceansar_amd import configs
     d = tempfile.mkdtemp(prefix=tag)
     t = time.perf_counter()
-    scenes = [configs.load_scene(configs.cfg_text(**kw), d, name='s%d.cfg' % i)
+    scenes = [configs.load_scene(configs.cfg_text(**kw), d, name='s%d.cfg' % i,
+                                 device_init=device_init)
               for i, kw in enumerate(kws)]
     return scenes, time.perf_counter() - t
 
@@ -219,7 +222,8 @@ def bench_serve(args):
     for kw in kws:
         if rng.uniform() < 0.1:
             kw.update(L=2048., dx=1., num_ch=2, b_ati=[0., 5.])
-    scenes, t_host = _scenes(kws, 'serve')
+    # job surfaces are built on the device: their spectra never leave HBM
+    scenes, t_host = _scenes(kws, 'serve', device_init=None if args.host_init else 0)
     W = sum(sc.contributions for sc in scenes)
     # plain capacity: the jobs back to back (every job arrived at t = 0)
     warm = [serve.Job(i, sc, 0.) for i, sc in enumerate(scenes[:2])]
@@ -248,6 +252,7 @@ def bench_serve(args):
         'p99_speedup_vs_plain': plain['p99_latency_s'] / got['p99_latency_s'],
         'throughput_vs_plain': got['contributions_per_s'] / plain['contributions_per_s'],
         'plain_capacity': capacity, 'host_scene_setup_s': t_host,
+        'surface_init': 'host numpy' if args.host_init else 'device (osar_surface_init)',
     }
 
 
@@ -282,6 +287,8 @@ def main():
                          'arrivals, p50/p99 latency vs sequential plain execution')
     ap.add_argument('--jobs', type=int, default=None, help='scenes (batch) or jobs (serve)')
     ap.add_argument('--streams', type=int, default=4)
+    ap.add_argument('--host-init', action='store_true',
+                    help='serve form: build the job surfaces on the host instead of the device')
     ap.add_argument('--copy-probe-mib', type=int, default=0,
                     help='after timing, run one device copy of this many MiB (a known-size '
                          'kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters)')
