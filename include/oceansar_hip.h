@@ -150,6 +150,33 @@ typedef struct osar_echo_desc {
 
 int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void* stream);
 
+/* Persistent batched accumulation: the launches of n descriptors (scenes or
+   chunks of any sizes, 1 to 4 channels each; one polarisation mode and one
+   pulses_per_block, 4 or 8, for all; no specular or cross-track terms) in ONE
+   persistent launch. No counterpart in the reference (which runs one scene per
+   process); it replaces n osar_echo_accumulate calls with identical results
+   up to the float32 order of the raw-cube additions. The grid holds as many
+   workgroups as fit the device at once (or `workgroups`); each takes work
+   items (P pulses x R rows x a column chunk of one descriptor) from a queue
+   per XCD until the queue is empty, so small and large scenes share the
+   device. The plan (descriptors, item tables, zeroed counters) is written by
+   the call into the caller's pinned host buffer and copied to the caller's
+   device buffer on the stream: both must stay untouched until the stream
+   has run the launch (for a captured graph: as long as the graph is
+   replayed); concurrent batches need their own plan buffers. */
+#define OSAR_ECHO_BATCH_MAX 4096
+typedef struct osar_echo_batch_desc {
+  int32_t n;                    /* descriptors, 1 .. OSAR_ECHO_BATCH_MAX */
+  int32_t workgroups;           /* grid size, 0: as many as fit the device at once */
+  const osar_echo_desc* descs;  /* host array [n], checked as osar_echo_accumulate checks */
+  void* plan_host;              /* pinned host buffer of osar_echo_batch_plan_bytes(n) */
+  void* plan_dev;               /* device buffer of the same size */
+} osar_echo_batch_desc;
+
+/* Plan bytes of an n-descriptor batch, -1 for n out of range. */
+int64_t osar_echo_batch_plan_bytes(int32_t n);
+int osar_echo_accumulate_batch(osar_ctx* ctx, const osar_echo_batch_desc* b, void* stream);
+
 /* The same accumulation with every value FP64 (the inputs the reference keeps
    in float32 aside: surface fields and the range-response table), into raw64:
    the precision reference of osar_echo_accumulate, at the same call site

@@ -67,6 +67,14 @@ class SurfaceDesc(ctypes.Structure):
         [('out', c_void_p * NUM_FIELDS)]
 
 
+class EchoBatchDesc(ctypes.Structure):
+    _fields_ = [('n', c_int32), ('workgroups', c_int32), ('descs', c_void_p),
+                ('plan_host', c_void_p), ('plan_dev', c_void_p)]
+
+
+ECHO_BATCH_MAX = 4096   # OSAR_ECHO_BATCH_MAX
+
+
 class InitDesc(ctypes.Structure):
     _fields_ = [(n, c_int32) for n in ('nx', 'ny', 'spread', 'swell')] + \
         [('seed', ctypes.c_uint64)] + \
@@ -93,7 +101,8 @@ EXPORTS = ['osar_ctx_create', 'osar_ctx_destroy', 'osar_last_error', 'osar_versi
            'osar_fill_zero_c32',
            'osar_echo_deposit', 'osar_facet_bragg', 'osar_fill_zero_f64',
            'osar_surface_workspace_bytes', 'osar_surface_synth', 'osar_surface_synth_noalloc',
-           'osar_sar_focus', 'osar_surface_init']
+           'osar_sar_focus', 'osar_surface_init', 'osar_echo_batch_plan_bytes',
+           'osar_echo_accumulate_batch']
 SYNTH_SCRATCH_BYTES = 256 << 20    # OSAR_SYNTH_SCRATCH_BYTES
 
 _lib = None
@@ -127,6 +136,9 @@ def load():
     lib.osar_surface_synth_noalloc.argtypes = [c_void_p, POINTER(SurfaceDesc), c_void_p]
     lib.osar_sar_focus.argtypes = [c_void_p, POINTER(FocusDesc), c_void_p]
     lib.osar_surface_init.argtypes = [c_void_p, POINTER(InitDesc), c_void_p]
+    lib.osar_echo_batch_plan_bytes.argtypes = [c_int32]
+    lib.osar_echo_batch_plan_bytes.restype = c_int64
+    lib.osar_echo_accumulate_batch.argtypes = [c_void_p, POINTER(EchoBatchDesc), c_void_p]
     for name, cls in ((0, EchoDesc), (1, BraggDesc), (2, SurfaceDesc), (3, FocusDesc),
                       (4, InitDesc)):
         n = lib.osar_struct_size(name)

@@ -341,3 +341,165 @@ def workspace_bytes(sc, chunk=32):
     per = spectra_bytes(scene_fields(sc), sc.surface.choppy,
                         sc.bragg_enable and not sc.spec_enable)
     return int(per * (chunk + 1) * n2 + 8 * n2)
+
+
+class PersistentBatch:
+    """The batching form as persistent launches (osar_echo_accumulate_batch).
+
+    The scenes advance in rounds of one chunk each: the synthesis and Bragg
+    refreshes of every scene's next chunk (spread over `n_streams` streams,
+    one library context each), then ONE persistent accumulator launch over the
+    chunks of all scenes, whose workgroups drain per-XCD queues of work items
+    largest descriptor first. Chunk buffers are double-buffered, so a round's
+    synthesis and Bragg run beside the previous round's accumulation. All
+    scenes share P = 8 pulses per workgroup where every scene's shape fits two
+    workgroups per CU (else 4); each item runs the accumulator variant of its
+    scene's channel count. `capture()` records
+    every round into one HIP graph. A scene's raw cube equals its plain run's
+    up to the float32 order of the raw-cube additions.
+    """
+
+    def __init__(self, scenes, n_streams=4, device=0, chunk=32, workgroups=0):
+        import torch
+        self.torch = torch
+        self.scenes = list(scenes)
+        self.workgroups = int(workgroups)
+        for sc in self.scenes:
+            if sc.spec_enable or np.any(np.atleast_1d(sc.b_xti) != 0) or sc.num_ch > 4:
+                raise ValueError('persistent batches take Bragg-only scenes without '
+                                 'cross-track baselines, 1 to 4 channels')
+        if len({sc.pol for sc in self.scenes}) != 1:
+            raise ValueError('persistent batches take one polarisation mode')
+        self.device = device
+        self.n_streams = max(1, min(int(n_streams), len(self.scenes)))
+        self.ctxs = [_lib.Context(device) for _ in range(self.n_streams)]
+        self.streams = [torch.cuda.Stream(device=device) for _ in range(self.n_streams)]
+        self.ctx = _lib.context(device)
+        order = sorted(range(len(self.scenes)), key=lambda i: -contributions(self.scenes[i]))
+        self.slot = [0] * len(self.scenes)
+        load = [0.] * self.n_streams
+        for i in order:
+            s = int(np.argmin(load))
+            self.slot[i] = s
+            load[s] += contributions(self.scenes[i])
+        self.ds = [DeviceScene(sc, device=device, ctx=self.ctxs[self.slot[i]])
+                   for i, sc in enumerate(self.scenes)]
+        shapes = None
+        for P in (8, 4):
+            shapes = [ds.default_shape(pulses=(P,)) for ds in self.ds]
+            if all(s.pulses_per_block == P for s in shapes):
+                break
+        if len({s.pulses_per_block for s in shapes}) != 1:
+            raise ValueError('no common pulses_per_block (4 or 8) for the batch')
+        for ds, s in zip(self.ds, shapes):
+            ds.shape = s
+        self.chunk = int(chunk)
+        # two chunk buffer sets per scene: round r + 1's synthesis and Bragg
+        # (buffer set (r + 1) mod 2) run beside round r's accumulation
+        self.states = [ds.start(0, None, self.chunk, nbuf=2) for ds in self.ds]
+        self.plans = []           # (pinned host, device) plan buffers per round
+        self.graph = None
+
+    @property
+    def contributions(self):
+        return sum(contributions(sc) for sc in self.scenes)
+
+    def _plan(self, r, n):
+        torch = self.torch
+        nb = int(self.ctx.lib.osar_echo_batch_plan_bytes(n))
+        if nb < 0:
+            raise ValueError('batch of %d descriptors out of range' % n)
+        while len(self.plans) <= r:
+            self.plans.append(None)
+        have = self.plans[r]
+        if have is None or have[0].numel() < nb:
+            host = torch.empty(nb, dtype=torch.uint8, pin_memory=True)
+            dev = torch.empty(nb, dtype=torch.uint8, device=torch.device('cuda', self.device))
+            self.plans[r] = (host, dev)
+        return self.plans[r]
+
+    def _enqueue(self):
+        """Every round on the streams (synthesis, Bragg) and the current stream
+        (one batched accumulation per round)."""
+        import ctypes
+        torch = self.torch
+        cur = torch.cuda.current_stream(self.device)
+        for ds, st in zip(self.ds, self.states):
+            st.p = st.p0
+            st.held = -1
+            st.snap_held = -1
+            ds.zero_c32(st.raw)
+            ds.zero_f64(st.rcs_sum)
+        for st in self.states:
+            st.k = 0
+        for s in self.streams:
+            s.wait_stream(cur)          # the zeroing above
+        done = [None, None]             # accumulation that last read buffer set r mod 2
+        r = 0
+        while True:
+            if done[r % 2] is not None:
+                for s in self.streams:
+                    s.wait_event(done[r % 2])
+            descs = []
+            for i, (ds, st) in enumerate(zip(self.ds, self.states)):
+                with torch.cuda.stream(self.streams[self.slot[i]]):
+                    d = ds.prepare(st)
+                if d is not None:
+                    descs.append(d)
+            if not descs:
+                break
+            for s in self.streams:
+                cur.wait_stream(s)
+            # plan order = queue order: the largest launches first
+            descs.sort(key=lambda d: -(d.npulses * d.nx * d.ny))
+            arr = (_lib.EchoDesc * len(descs))(*descs)
+            host, dev = self._plan(r, len(descs))
+            b = _lib.EchoBatchDesc()
+            b.n, b.workgroups = len(descs), self.workgroups
+            b.descs = ctypes.cast(arr, ctypes.c_void_p)
+            b.plan_host, b.plan_dev = host.data_ptr(), dev.data_ptr()
+            _lib.check(self.ctx.lib.osar_echo_accumulate_batch(self.ctx.h, ctypes.byref(b),
+                                                               _lib.stream_ptr()))
+            ev = torch.cuda.Event()
+            ev.record(cur)
+            done[r % 2] = ev
+            r += 1
+        for s in self.streams:
+            cur.wait_stream(s)
+        self.rounds = r
+
+    def run(self):
+        """Eager run of every round; returns the makespan in seconds."""
+        torch = self.torch
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        self._enqueue()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    def capture(self):
+        """Capture every round into one graph (after an eager run that sizes the
+        workspaces and sets the kernels' LDS limits)."""
+        torch = self.torch
+        self.run()
+        g = torch.cuda.CUDAGraph()
+        cap = torch.cuda.Stream(device=self.device)
+        cap.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.graph(g, stream=cap):
+            self._enqueue()
+        torch.cuda.synchronize()
+        self.graph = g
+        return g
+
+    def replay(self):
+        torch = self.torch
+        if self.graph is None:
+            self.capture()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        self.graph.replay()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    def results(self):
+        return [ds.finish(st) for ds, st in zip(self.ds, self.states)]

@@ -28,6 +28,7 @@
 // FP64 before the FP32 sin/cos; scene values, channel factors and the
 // accumulation are FP32 (LDS partial sums FP64).
 #include <limits.h>
+#include <string.h>
 
 #include "common.h"
 
@@ -363,19 +364,15 @@ __device__ __forceinline__ void flush_slot(const Lines& L, const float2 (&s)[NPO
 
 // XTI: the cross-track phase varies per facet, so channels cannot share a
 // window; this template variant deposits every tap of every channel.
-// Work item w = (pulse group fastest, then row tile) is remapped with
-// xcd_item so that each XCD runs a contiguous range of items: the pulse groups
-// of one snapshot interval, which read the same surface rows, then share an L2.
+// Items run so that each XCD takes a contiguous range: the pulse groups of one
+// snapshot interval, which read the same surface rows, then share an L2.
 
+// One work item of the accumulator: P pulses x R rows x a column chunk of
+// descriptor d, in the LDS of the calling workgroup (every thread calls it).
 template <int P, int NCH, int NPOL, bool SPEC, bool XTI>
-// Two workgroups per CU (4 waves per SIMD, at most 128 VGPRs) for the plain
-// Bragg kernels when the LDS lines fit twice (column chunks make them fit);
-// the specular and XTI variants keep their registers (2 waves per SIMD).
-__global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumulate_kernel(osar_echo_desc d, int W,
-                                                                   int R, int LS, int n_pgroups,
-                                                                   int n_xchunks, int CX,
-                                                                   int n_items) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
+__device__ __forceinline__ void echo_item(const osar_echo_desc& d, unsigned char* lds_raw,
+                                          int item, int W, int R, int LS, int n_pgroups,
+                                          int n_xchunks, int CX) {
   constexpr int NL = NPOL * NCH * P;
   double* acc = (double*)lds_raw;                  // [NL][LS]
   float* tab = (float*)(acc + (size_t)NL * LS);    // [kTabRows][kTaps]
@@ -383,7 +380,6 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
 
   const int tid = threadIdx.x;
   // item = (pulse group fastest, then column chunk, then row tile)
-  const int item = osar::xcd_item(blockIdx.x, n_items);
   const int pstart = d.pulse0 + (item % n_pgroups) * P;
   const int pend = d.pulse0 + d.npulses;
   const int rest = item / n_pgroups;
@@ -670,6 +666,72 @@ __global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumula
   }
 }
 
+// Two workgroups per CU (4 waves per SIMD, at most 128 VGPRs) for the plain
+// Bragg kernels when the LDS lines fit twice (column chunks make them fit);
+// the specular and XTI variants keep their registers (2 waves per SIMD).
+// Work item w = (pulse group fastest, then column chunk, then row tile) is
+// remapped with xcd_item so that each XCD runs a contiguous range of items.
+template <int P, int NCH, int NPOL, bool SPEC, bool XTI>
+__global__ __launch_bounds__(kThreads, (SPEC || XTI) ? 2 : 4) void echo_accumulate_kernel(
+    osar_echo_desc d, int W, int R, int LS, int n_pgroups, int n_xchunks, int CX, int n_items) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  echo_item<P, NCH, NPOL, SPEC, XTI>(d, lds_raw, osar::xcd_item(blockIdx.x, n_items), W, R, LS,
+                                     n_pgroups, n_xchunks, CX);
+}
+
+// Persistent batched accumulation (osar_echo_accumulate_batch): the launch
+// parameters of one descriptor of the batch.
+struct BatchParams {
+  int W, R, LS, n_pgroups, n_xchunks, CX, n_items, reserved;
+};
+constexpr int kBatchCounterStride = 32;   // ints between XCD counters: one 128-byte line each
+
+// The grid holds as many workgroups as fit the device at once; workgroup b
+// runs on XCD b mod 8 and takes items from that XCD's queue: the concatenation,
+// over the descriptors in plan order, of the contiguous eighth of each
+// descriptor's items that xcd_item gives XCD x, so items sharing surface rows
+// still run behind one L2. qstart[x][k] is the queue position of descriptor
+// k's part; a workgroup exits when its queue is empty.
+// Each item runs the accumulator variant of its own descriptor's channel count.
+template <int P, int NPOL>
+__global__ __launch_bounds__(kThreads, 4) void echo_batch_kernel(
+    const osar_echo_desc* __restrict__ descs, const BatchParams* __restrict__ prm,
+    const int* __restrict__ qstart, int* counters, int n) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ int s_item;
+  const int x = blockIdx.x % osar::kXcds;
+  const int* qs = qstart + (size_t)x * (n + 1);
+  const int total = qs[n];
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(&counters[x * kBatchCounterStride], 1);
+    __syncthreads();
+    const int c = s_item;
+    __syncthreads();   // every thread has read s_item before thread 0 takes the next
+    if (c >= total) return;
+    int lo = 0, hi = n - 1;   // the descriptor whose part holds queue position c
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (qs[mid] <= c) lo = mid;
+      else hi = mid - 1;
+    }
+    const int k = __builtin_amdgcn_readfirstlane(lo);
+    const BatchParams p = prm[k];
+    const int q = p.n_items / osar::kXcds, r = p.n_items % osar::kXcds;
+    const int item = x * q + min(x, r) + (c - qs[k]);
+    const osar_echo_desc& d = descs[k];
+    if (d.n_ch <= 1)
+      echo_item<P, 1, NPOL, false, false>(d, lds_raw, item, p.W, p.R, p.LS, p.n_pgroups,
+                                          p.n_xchunks, p.CX);
+    else if (d.n_ch <= 2)
+      echo_item<P, 2, NPOL, false, false>(d, lds_raw, item, p.W, p.R, p.LS, p.n_pgroups,
+                                          p.n_xchunks, p.CX);
+    else
+      echo_item<P, 4, NPOL, false, false>(d, lds_raw, item, p.W, p.R, p.LS, p.n_pgroups,
+                                          p.n_xchunks, p.CX);
+    __syncthreads();   // the item's LDS lines are flushed before the next item clears them
+  }
+}
+
 size_t echo_lds_bytes(int P, int nch_c, int npol, int LS) {
   return sizeof(double) * (size_t)npol * nch_c * P * LS + sizeof(float) * kTabLds +
          sizeof(int) * P;
@@ -891,8 +953,10 @@ __global__ __launch_bounds__(kFp64Threads) void echo_fp64_kernel(osar_echo_desc 
 
 }  // namespace
 
-extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void* stream) {
-  OSAR_PTR(ctx);
+namespace {
+
+// The checks of osar_echo_accumulate; P, W, R: the launch shape with defaults.
+int check_echo_desc(const osar_echo_desc* d, int* P_out, int* W_out, int* R_out) {
   OSAR_PTR(d);
   OSAR_REQUIRE(d->nx > 0 && d->ny > 0 && d->nbins > 0, OSAR_EINVAL, "empty scene");
   OSAR_REQUIRE(d->n_ch >= 1 && d->n_ch <= OSAR_MAX_CH, OSAR_EINVAL, "n_ch must be 1..%d",
@@ -903,6 +967,12 @@ extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void
                    d->pulse0 + d->npulses <= d->n_pulses_total,
                OSAR_EINVAL, "pulse window [%d, %d) outside [0, %d)", d->pulse0,
                d->pulse0 + d->npulses, d->n_pulses_total);
+  int P = d->pulses_per_block ? d->pulses_per_block : 8;
+  const int R = d->rows_per_block ? d->rows_per_block : 64;
+  int W = d->window_bins ? d->window_bins : 512;
+  *P_out = P;
+  *W_out = W;
+  *R_out = R;
   if (d->npulses == 0) return OSAR_OK;
   // Every snapshot the window touches must be resident.
   const int m_first = d->pulse0 / d->K;
@@ -944,27 +1014,160 @@ extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void
   }
   OSAR_REQUIRE(d->pol >= 0 && d->pol <= 2 && (d->pol == 2) == (d->n_pol == 2), OSAR_EINVAL,
                "pol %d inconsistent with n_pol %d", d->pol, d->n_pol);
-  int P = d->pulses_per_block ? d->pulses_per_block : 8;
-  const int R = d->rows_per_block ? d->rows_per_block : 64;
-  int W = d->window_bins ? d->window_bins : 512;
   OSAR_REQUIRE(R >= 1 && R <= 1024 && W >= 8 && W <= 65536 && d->cols_per_block >= 0,
                OSAR_EINVAL, "bad launch shape R=%d W=%d cols=%d", R, W, d->cols_per_block);
+  OSAR_REQUIRE(P == 2 || P == 4 || P == 8 || P == 16, OSAR_EINVAL,
+               "pulses_per_block must be 2, 4, 8 or 16 (got %d)", P);
   // field inputs are addressed by 32-bit byte offsets from a pulse group's
   // first snapshot (its lanes are at most P / K + 2 snapshots further on)
   OSAR_REQUIRE((uint64_t)d->nx * d->ny * 4 * (uint64_t)(P / d->K + 3) < (1ull << 32),
                OSAR_EUNSUPPORTED, "surface of %d x %d facets too large for 32-bit field offsets",
                d->nx, d->ny);
+  return OSAR_OK;
+}
+
+// Byte layout of a batch plan: descriptors, parameters, XCD queue starts, counters.
+struct PlanLayout {
+  size_t descs, prm, qs, cnt, total;
+};
+PlanLayout plan_layout(int n) {
+  auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  PlanLayout L;
+  L.descs = 0;
+  L.prm = up(sizeof(osar_echo_desc) * (size_t)n);
+  L.qs = L.prm + up(sizeof(BatchParams) * (size_t)n);
+  L.cnt = L.qs + up(sizeof(int) * (size_t)osar::kXcds * (n + 1));
+  L.total = L.cnt + sizeof(int) * osar::kXcds * kBatchCounterStride;
+  return L;
+}
+
+const void* batch_kernel_for(int P, int npol) {
+  static const void* const k[4] = {
+      (const void*)echo_batch_kernel<8, 1>, (const void*)echo_batch_kernel<8, 2>,
+      (const void*)echo_batch_kernel<4, 1>, (const void*)echo_batch_kernel<4, 2>};
+  return k[(P == 8 ? 0 : 2) + (npol - 1)];
+}
+
+}  // namespace
+
+extern "C" int osar_echo_accumulate(osar_ctx* ctx, const osar_echo_desc* d, void* stream) {
+  OSAR_PTR(ctx);
+  int P = 0, W = 0, R = 0;
+  const int rc = check_echo_desc(d, &P, &W, &R);
+  if (rc) return rc;
+  if (d->npulses == 0) return OSAR_OK;
   hipStream_t s = (hipStream_t)stream;
   OSAR_HIP(hipSetDevice(ctx->device));
   switch (P) {
     case 2: return launch_ch<2>(ctx, d, W, R, s);
     case 4: return launch_ch<4>(ctx, d, W, R, s);
     case 8: return launch_ch<8>(ctx, d, W, R, s);
-    case 16: return launch_ch<16>(ctx, d, W, R, s);
-    default: break;
+    default: return launch_ch<16>(ctx, d, W, R, s);
   }
-  osar_set_error("pulses_per_block must be 2, 4, 8 or 16 (got %d)", P);
-  return OSAR_EINVAL;
+}
+
+extern "C" int64_t osar_echo_batch_plan_bytes(int32_t n) {
+  if (n < 1 || n > OSAR_ECHO_BATCH_MAX) return -1;
+  return (int64_t)plan_layout(n).total;
+}
+
+extern "C" int osar_echo_accumulate_batch(osar_ctx* ctx, const osar_echo_batch_desc* b,
+                                          void* stream) {
+  OSAR_PTR(ctx);
+  OSAR_PTR(b);
+  OSAR_REQUIRE(b->n >= 1 && b->n <= OSAR_ECHO_BATCH_MAX, OSAR_EINVAL,
+               "batch of %d descriptors: 1 .. %d", b->n, OSAR_ECHO_BATCH_MAX);
+  OSAR_PTR(b->descs);
+  OSAR_PTR(b->plan_host);
+  OSAR_PTR(b->plan_dev);
+  OSAR_REQUIRE(b->workgroups >= 0, OSAR_EINVAL, "negative workgroup count");
+  const int n = b->n;
+  const PlanLayout L = plan_layout(n);
+  unsigned char* h = (unsigned char*)b->plan_host;
+  BatchParams* prm = (BatchParams*)(h + L.prm);
+  int* qs = (int*)(h + L.qs);
+  int P0 = 0, npol = 0, pol = -1;
+  for (int k = 0; k < n; ++k) {
+    const osar_echo_desc* d = &b->descs[k];
+    int P = 0, W = 0, R = 0;
+    const int rc = check_echo_desc(d, &P, &W, &R);
+    if (rc) return rc;
+    OSAR_REQUIRE(!(d->flags & (OSAR_ECHO_SPECULAR | OSAR_ECHO_XTI)), OSAR_EUNSUPPORTED,
+                 "descriptor %d: specular and cross-track batches are not supported", k);
+    OSAR_REQUIRE(P == 4 || P == 8, OSAR_EUNSUPPORTED,
+                 "descriptor %d: batches take pulses_per_block 4 or 8 (got %d)", k, P);
+    OSAR_REQUIRE(d->n_ch <= 4, OSAR_EUNSUPPORTED, "descriptor %d: batches take at most 4 "
+                 "channels (got %d)", k, d->n_ch);
+    if (k == 0) {
+      P0 = P;
+      npol = d->n_pol;
+      pol = d->pol;
+    }
+    OSAR_REQUIRE(P == P0 && d->n_pol == npol && d->pol == pol, OSAR_EINVAL,
+                 "descriptor %d: every descriptor of a batch needs the same pulses_per_block "
+                 "and polarisation", k);
+    BatchParams& p = prm[k];
+    p.W = W;
+    p.R = R;
+    p.LS = line_stride(W);
+    p.n_pgroups = (d->npulses + P - 1) / P;
+    p.CX = (d->cols_per_block > 0 && d->cols_per_block < d->nx) ? d->cols_per_block : d->nx;
+    p.n_xchunks = (d->nx + p.CX - 1) / p.CX;
+    const int64_t items = d->npulses == 0 ? 0
+                                          : (int64_t)p.n_pgroups * p.n_xchunks *
+                                                ((d->ny + R - 1) / R);
+    OSAR_REQUIRE(items < INT_MAX / 2, OSAR_EINVAL, "descriptor %d: too many items", k);
+    p.n_items = (int)items;
+    p.reserved = 0;
+  }
+  size_t lds = 0;   // the largest item's lines (each item runs its own channel count)
+  for (int k = 0; k < n; ++k) {
+    const int c = b->descs[k].n_ch;
+    const size_t l = echo_lds_bytes(P0, c <= 1 ? 1 : c <= 2 ? 2 : 4, npol, prm[k].LS);
+    if (l > lds) lds = l;
+  }
+  OSAR_REQUIRE(lds <= ctx->max_lds, OSAR_EUNSUPPORTED,
+               "batch window needs %zu B of LDS (limit %zu): lower window_bins", lds,
+               ctx->max_lds);
+  // queue starts: XCD x takes the eighth [x q + min(x, r), ...) of every descriptor
+  int64_t total = 0;
+  for (int x = 0; x < osar::kXcds; ++x) {
+    int64_t pos = 0;
+    for (int k = 0; k < n; ++k) {
+      qs[x * (n + 1) + k] = (int)pos;
+      const int q = prm[k].n_items / osar::kXcds, r = prm[k].n_items % osar::kXcds;
+      pos += q + (x < r ? 1 : 0);
+    }
+    OSAR_REQUIRE(pos < INT_MAX / 2, OSAR_EINVAL, "batch too large");
+    qs[x * (n + 1) + n] = (int)pos;
+    total += pos;
+  }
+  memcpy(h + L.descs, b->descs, sizeof(osar_echo_desc) * (size_t)n);
+  memset(h + L.cnt, 0, L.total - L.cnt);
+  if (total == 0) return OSAR_OK;
+  hipStream_t s = (hipStream_t)stream;
+  OSAR_HIP(hipSetDevice(ctx->device));
+  // the plan (zeroed counters included) reaches the device on the stream
+  OSAR_HIP(hipMemcpyAsync(b->plan_dev, b->plan_host, L.total, hipMemcpyHostToDevice, s));
+  const void* kern = batch_kernel_for(P0, npol);
+  {
+    const int rc = osar_reserve_lds(kern, lds);
+    if (rc) return rc;
+  }
+  // launch bounds of 4 waves per SIMD: two 8-wave workgroups per CU when their LDS fits
+  const int per_cu = lds * 2 <= ctx->max_lds ? 2 : 1;
+  int G = b->workgroups > 0 ? b->workgroups : per_cu * ctx->n_cu;
+  G = G < osar::kXcds ? osar::kXcds : (G / osar::kXcds) * osar::kXcds;
+  unsigned char* dv = (unsigned char*)b->plan_dev;
+  const osar_echo_desc* descs = (const osar_echo_desc*)(dv + L.descs);
+  const BatchParams* prm_d = (const BatchParams*)(dv + L.prm);
+  const int* qs_d = (const int*)(dv + L.qs);
+  int* cnt_d = (int*)(dv + L.cnt);
+  int nn = n;
+  void* args[] = {&descs, &prm_d, &qs_d, &cnt_d, &nn};
+  OSAR_HIP(hipLaunchKernel(kern, dim3(G), dim3(kThreads), args, lds, s));
+  OSAR_LAUNCHED("echo_batch_kernel");
+  return OSAR_OK;
 }
 
 extern "C" int osar_echo_deposit(osar_ctx* ctx, const double* srg, const osar_c32* scene,

@@ -280,26 +280,30 @@ class DeviceScene:
         LS = ((2 * W + 31) // 32) * 32 + 4            # FP64 lines
         return 8 * npol * nch_c * P * LS + 4 * 168 + 4 * P   # lines, tap table, window starts
 
-    def default_shape(self, lds_budget=160 * 1024):
+    def default_shape(self, lds_budget=160 * 1024, n_ch=None, pulses=(8, 4)):
         """The plain kernels are built for two workgroups per CU (4 waves per SIMD):
         the largest P in (8, 4) and the widest column chunk whose LDS lines fit
         twice in a CU (P = 8 loads each facet once for 8 pulses); otherwise
-        (specular, XTI, or no fit) the largest P whose lines fit once."""
+        (specular, XTI, or no fit) the largest P whose lines fit once. n_ch:
+        size the lines for this many channels (a batch kernel built for the
+        batch's largest channel count) instead of the scene's; pulses: the P
+        values tried for two workgroups per CU."""
         sc = self.sc
+        nch = sc.num_ch if n_ch is None else int(n_ch)
         if not (sc.spec_enable or np.any(sc.b_xti != 0)):
-            for P in (8, 4):
+            for P in pulses:
                 R = min(8 * (64 // P), sc.Ny)
                 for cols in (sc.Nx, sc.Nx // 2, sc.Nx // 4, sc.Nx // 8):
                     if cols < 64 or (cols < sc.Nx and not self.sr_monotonic):
                         break
                     c = cols if cols < sc.Nx else 0
                     W = self.window_bins(R, c)
-                    if self.lds_bytes(P, sc.num_ch, self.npol, W) <= lds_budget // 2:
+                    if self.lds_bytes(P, nch, self.npol, W) <= lds_budget // 2:
                         return LaunchShape(P, R, W, c)
         for P in (8, 4, 2):
             R = min(8 * (64 // P), sc.Ny)      # 8 waves x 64/P rows
             W = self.window_bins(R)
-            if self.lds_bytes(P, sc.num_ch, self.npol, W) <= lds_budget:
+            if self.lds_bytes(P, nch, self.npol, W) <= lds_budget:
                 return LaunchShape(P, R, W)
         raise RuntimeError('range window of %d bins does not fit the LDS even at P=2' % W)
 
@@ -504,8 +508,10 @@ class DeviceScene:
         return self.finish(st)
 
     def start(self, p0=0, p1=None, chunk_snaps=32, raw=None, shape=None, overlap=False,
-              fp64=False):
-        """Allocate the buffers of a run over pulses [p0, p1) (no launches)."""
+              fp64=False, nbuf=None):
+        """Allocate the buffers of a run over pulses [p0, p1) (no launches).
+        nbuf: chunk buffer sets (default 2 with overlap, else 1); with two,
+        consecutive prepare() calls alternate between them."""
         import torch
         sc = self.sc
         st = RunState()
@@ -519,7 +525,9 @@ class DeviceScene:
         st.raw = self.new_raw(fp64) if raw is None else raw
         st.names = self.fields_needed()
         nx, ny = sc.Nx, sc.Ny
-        nbuf = 2 if overlap else 1
+        nbuf = (2 if overlap else 1) if nbuf is None else int(nbuf)
+        if nbuf not in (1, 2) or (overlap and nbuf != 2):
+            raise ValueError('nbuf must be 1 or 2 (2 with overlap)')
         st.bufs = [{n: torch.empty((st.C + 1, nx, ny), dtype=torch.float32, device=self.dev)
                     for n in st.names} for _ in range(nbuf)]
         nref = len(sc.rcs_refresh) if sc.bragg_enable else 0
@@ -566,10 +574,30 @@ class DeviceScene:
         st.echo_done[k] = done
         return True
 
-    def _step(self, st, timer, sa, sb):
+    def prepare(self, st):
+        """Synthesis and Bragg of the next chunk of run `st` on the current
+        stream; returns the accumulator descriptor of that chunk, not launched
+        (for osar_echo_accumulate_batch), or None when the run is done. The next
+        prepare reuses the chunk's buffers: the accumulation must be on the
+        stream before it."""
+        if st.p >= st.p1:
+            return None
+        if st.streams is not None or st.fp64:
+            raise ValueError('prepare() runs the single-stream mixed-precision path')
+        self._step(st, None, None, None, defer=True)
+        p, q1, m0, r0 = st.chunk
+        buf, amp = st.bufs[st.k], st.amps[st.k]
+        d = self.echo_desc(buf['Dx'], buf['Dy'], buf['Dz'], m0, amp, r0, p, q1 - p, st.raw,
+                           st.shape, spec=buf if self.sc.spec_enable else None,
+                           spec_energy=st.spec_energy)
+        if len(st.bufs) > 1:
+            st.k = 1 - st.k
+        return d
+
+    def _step(self, st, timer, sa, sb, defer=False):
         """Synthesis and Bragg of the next chunk into buffer set st.k (plus the
-        accumulation when not overlapped). Returns the event the accumulation
-        waits for (overlap) or None."""
+        accumulation when neither overlapped nor deferred). Returns the event the
+        accumulation waits for (overlap) or None."""
         import torch
         sc = self.sc
         K = sc.K
@@ -627,7 +655,8 @@ class DeviceScene:
         st.chunk = (p, q1, m0, r0)
         st.p = q1
         if sa is None:
-            self._step_echo(st, timer, None)
+            if not defer:
+                self._step_echo(st, timer, None)
             return None
         ready = torch.cuda.Event()
         ready.record(sa)

@@ -53,6 +53,35 @@ def test_batch_modes_agree(need_gpu, tmp_path_factory):
         torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize('workgroups', [0, 8])
+def test_persistent_batch_agrees_with_plain(need_gpu, tmp_path_factory, workgroups):
+    """osar_echo_accumulate_batch: one persistent accumulator launch per round
+    over the chunks of every scene (1-3 channels, uneven sizes, several
+    rounds; a grid of 8 workgroups makes each drain long queues), eager and
+    as graph replays with eager work between them."""
+    import torch
+    from oceansar_amd.batch import BatchRunner, PersistentBatch
+    scenes = _mixed_scenes(tmp_path_factory, 5, seed=13)
+    br = BatchRunner(scenes, n_streams=1)
+    br.run_sequential()
+    ref = _host(br.results())
+    nrcs_ref = [n for _, n in br.results()]
+    pb = PersistentBatch(scenes, n_streams=3, chunk=4, workgroups=workgroups)
+    pb.run()
+    assert pb.rounds > 1
+    for i, got in enumerate(_host(pb.results())):
+        assert rel_rms(got, ref[i]) < 1e-6, ('persistent', i)
+        np.testing.assert_allclose(pb.results()[i][1], nrcs_ref[i], rtol=1e-9)
+    pb.capture()
+    for rep in range(3):
+        pb.replay()
+        got = _host(pb.results())
+        for i in range(len(scenes)):
+            assert rel_rms(got[i], ref[i]) < 1e-6, ('persistent graph replay', rep, i)
+        br.run_sequential()
+        torch.cuda.synchronize()
+
+
 def test_batched_scene_matches_oracle(need_gpu, tmp_path_factory):
     import oracle.c_oracle as co
     import oracle.sar as osar
