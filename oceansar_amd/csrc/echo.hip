@@ -692,8 +692,9 @@ constexpr int kBatchCounterStride = 32;   // ints between XCD counters: one 128-
 // descriptor's items that xcd_item gives XCD x, so items sharing surface rows
 // still run behind one L2. qstart[x][k] is the queue position of descriptor
 // k's part; a workgroup exits when its queue is empty.
-// Each item runs the accumulator variant of its own descriptor's channel count.
-template <int P, int NPOL>
+// Each item runs the accumulator variant of its own descriptor's channel
+// count; NCHF > 0: every descriptor has NCHF (rounded) channels.
+template <int P, int NPOL, int NCHF>
 __global__ __launch_bounds__(kThreads, 4) void echo_batch_kernel(
     const osar_echo_desc* __restrict__ descs, const BatchParams* __restrict__ prm,
     const int* __restrict__ qstart, int* counters, int n) {
@@ -719,7 +720,10 @@ __global__ __launch_bounds__(kThreads, 4) void echo_batch_kernel(
     const int q = p.n_items / osar::kXcds, r = p.n_items % osar::kXcds;
     const int item = x * q + min(x, r) + (c - qs[k]);
     const osar_echo_desc& d = descs[k];
-    if (d.n_ch <= 1)
+    if (NCHF > 0)
+      echo_item<P, (NCHF > 0 ? NCHF : 1), NPOL, false, false>(d, lds_raw, item, p.W, p.R, p.LS,
+                                                              p.n_pgroups, p.n_xchunks, p.CX);
+    else if (d.n_ch <= 1)
       echo_item<P, 1, NPOL, false, false>(d, lds_raw, item, p.W, p.R, p.LS, p.n_pgroups,
                                           p.n_xchunks, p.CX);
     else if (d.n_ch <= 2)
@@ -1041,11 +1045,20 @@ PlanLayout plan_layout(int n) {
   return L;
 }
 
-const void* batch_kernel_for(int P, int npol) {
-  static const void* const k[4] = {
-      (const void*)echo_batch_kernel<8, 1>, (const void*)echo_batch_kernel<8, 2>,
-      (const void*)echo_batch_kernel<4, 1>, (const void*)echo_batch_kernel<4, 2>};
-  return k[(P == 8 ? 0 : 2) + (npol - 1)];
+template <int P, int NPOL>
+const void* batch_kernel_ch(int nchf) {
+  switch (nchf) {
+    case 1: return (const void*)echo_batch_kernel<P, NPOL, 1>;
+    case 2: return (const void*)echo_batch_kernel<P, NPOL, 2>;
+    case 4: return (const void*)echo_batch_kernel<P, NPOL, 4>;
+    default: return (const void*)echo_batch_kernel<P, NPOL, 0>;
+  }
+}
+
+// nchf: the rounded channel count every descriptor shares, 0 when they differ
+const void* batch_kernel_for(int P, int npol, int nchf) {
+  if (P == 8) return npol == 1 ? batch_kernel_ch<8, 1>(nchf) : batch_kernel_ch<8, 2>(nchf);
+  return npol == 1 ? batch_kernel_ch<4, 1>(nchf) : batch_kernel_ch<4, 2>(nchf);
 }
 
 }  // namespace
@@ -1121,10 +1134,12 @@ extern "C" int osar_echo_accumulate_batch(osar_ctx* ctx, const osar_echo_batch_d
     p.reserved = 0;
   }
   size_t lds = 0;   // the largest item's lines (each item runs its own channel count)
+  int nchf = -1;
   for (int k = 0; k < n; ++k) {
-    const int c = b->descs[k].n_ch;
-    const size_t l = echo_lds_bytes(P0, c <= 1 ? 1 : c <= 2 ? 2 : 4, npol, prm[k].LS);
+    const int c = b->descs[k].n_ch, cr = c <= 1 ? 1 : c <= 2 ? 2 : 4;
+    const size_t l = echo_lds_bytes(P0, cr, npol, prm[k].LS);
     if (l > lds) lds = l;
+    nchf = (nchf == -1 || nchf == cr) ? cr : 0;
   }
   OSAR_REQUIRE(lds <= ctx->max_lds, OSAR_EUNSUPPORTED,
                "batch window needs %zu B of LDS (limit %zu): lower window_bins", lds,
@@ -1149,7 +1164,7 @@ extern "C" int osar_echo_accumulate_batch(osar_ctx* ctx, const osar_echo_batch_d
   OSAR_HIP(hipSetDevice(ctx->device));
   // the plan (zeroed counters included) reaches the device on the stream
   OSAR_HIP(hipMemcpyAsync(b->plan_dev, b->plan_host, L.total, hipMemcpyHostToDevice, s));
-  const void* kern = batch_kernel_for(P0, npol);
+  const void* kern = batch_kernel_for(P0, npol, nchf);
   {
     const int rc = osar_reserve_lds(kern, lds);
     if (rc) return rc;
