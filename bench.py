@@ -33,6 +33,12 @@ if ROOT not in sys.path:
 
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 (vector, packed) peak
 HBM_PEAK_GBS = 8000.0
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    """One line per finished stage on stderr (the JSON line alone goes to stdout)."""
+    print('[bench %7.1f s] %s' % (time.perf_counter() - _T0, msg), file=sys.stderr, flush=True)
 
 
 def flop_per_contribution(n_ch, n_pol):
@@ -108,6 +114,7 @@ def cpu_baseline(sc, raw_gpu, sample_pulses, sample_pulses_1t):
     snaps = osar.SnapshotCache(sc, keep=n_snap + 2)
     tim = {}
     ref, _ = co.simulate_c(sc, p0, p1, snaps=snaps, threads=threads, timings=tim)
+    progress('cpu baseline: %d pulses at %d threads done' % (p1 - p0, threads))
     npol = len(sc.pols)
     per_pulse = sc.Nx * sc.Ny * sc.num_ch * npol
     w = per_pulse * (p1 - p0)
@@ -115,6 +122,7 @@ def cpu_baseline(sc, raw_gpu, sample_pulses, sample_pulses_1t):
     # one thread: the C loop only, over a shorter window of the same snapshots
     tim1 = {}
     co.simulate_c(sc, p0, p0 + sample_pulses_1t, snaps=snaps, threads=1, timings=tim1)
+    progress('cpu baseline: %d pulses at 1 thread done' % sample_pulses_1t)
     w1 = per_pulse * sample_pulses_1t
     # accuracy of the GPU run on the sample pulses (noise-free, uncalibrated raw)
     got = raw_gpu[:, :, p0:p1].cpu().numpy().astype(np.complex128)
@@ -256,9 +264,11 @@ def bench_serve(args):
     srv.serve([serve.Job(i, sc, 0.) for i, sc in enumerate(scenes[:2])])   # warm-up
     jobs = [serve.Job(i, sc, a) for i, (sc, a) in enumerate(zip(scenes, arrivals))]
     got = srv.serve(jobs)
+    progress('serving: %d jobs served' % n)
     seq_jobs = [serve.Job(i, sc, a) for i, (sc, a) in enumerate(zip(scenes, arrivals))]
     plain = serve.serve_sequential(seq_jobs)
     torch.cuda.synchronize()
+    progress('serving: the same arrivals on the plain path done')
     return {
         'metric': 'p99 scene latency (serving form: %d jobs, Poisson arrivals at 0.8 x plain capacity)' % n,
         'value': got['p99_latency_s'], 'unit': 's', 'n_gpus': 1, 'higher_is_better': False,
@@ -342,6 +352,7 @@ def main():
         ds.zero_c32(raw)
         ds.run(raw=raw, chunk_snaps=chunk, overlap=overlap)
     torch.cuda.synchronize()
+    progress('warm-up done (%d steps, launch group %d intervals)' % (args.warmup, chunk))
 
     stage_events = []
 
@@ -355,6 +366,7 @@ def main():
         ds.run(raw=raw, chunk_snaps=chunk, timer=timer, overlap=overlap)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
+    progress('timed steps done: %.1f ms per step' % (dt * 1e3))
 
     # the same steps ending with the raw cube on the host (pinned memory), the
     # scope "first launch to raw data on the host"
@@ -367,6 +379,7 @@ def main():
         host_raw.copy_(raw, non_blocking=True)
         torch.cuda.synchronize()
     dt_host = (time.perf_counter() - t0) / args.steps
+    progress('steps with the host copy done: %.1f ms per step' % (dt_host * 1e3))
 
     per_stage = {}
     echo_ms = []
@@ -439,6 +452,7 @@ def main():
         ref64, _ = ds.run(chunk_snaps=64, fp64=True)
         torch.cuda.synchronize()
         t_fp64 = time.perf_counter() - t0
+        progress('all-FP64 GPU run done (%.1f s)' % t_fp64)
         ref64 = ref64.cpu().numpy()
         e = float(np.linalg.norm(mixed - ref64) / np.linalg.norm(ref64))
         mag = np.abs(ref64)
