@@ -22,6 +22,12 @@
  *   osar_sar_focus       the processing loop of sar_focus for one channel
  *                        (oceansar/radarsim/sar_processor.py:131-225), the
  *                        consumer of the raw data
+ *   osar_insar_coregister  the azimuth sub-sample shift and flat-earth
+ *                        removal of insar_process
+ *                        (oceansar/radarsim/insar_processor.py:165-185)
+ *   osar_insar_multilook the multilooked intensities, interferograms and
+ *                        coherences of insar_process (:192-214), and the
+ *                        smoothing of ati_process (ati_processor.py:127-134)
  *
  * Conventions
  *   - Every pointer is a caller-owned device buffer; the library never frees
@@ -60,7 +66,8 @@ int osar_ctx_destroy(osar_ctx* ctx);
 const char* osar_last_error(void);
 /* Library version: major*10000 + minor*100 + patch. */
 int osar_version(void);
-/* sizeof of the descriptor structs (0 echo, 1 bragg, 2 surface, 3 focus, 4 init), so
+/* sizeof of the descriptor structs (0 echo, 1 bragg, 2 surface, 3 focus, 4 init,
+   5 coregistration, 6 multilook), so
    that bindings can check their layout; -1 for an unknown id. */
 int64_t osar_struct_size(int which);
 
@@ -385,6 +392,61 @@ typedef struct osar_focus_desc {
 } osar_focus_desc;
 
 int osar_sar_focus(osar_ctx* ctx, const osar_focus_desc* d, void* stream);
+
+/* ---- InSAR L1b processing ---------------------------------------------- */
+/* Sub-sample azimuth shift and flattening of one channel's SLC planes
+   (oceansar/radarsim/insar_processor.py:165-185): per range sample r and
+   plane, y = IFFT(FFT(x) S) along azimuth (lines of any length n_az), where
+   S is the reference's shift phasor. The host gives the spectrum kernel =
+   FFT_M(g) / M of the wrapped circular kernel g (g[j] = h[j], g[M - j] =
+   h[n_az - j], h = IFFT(S)), M = n_fft a power of two >= 2 n_az - 1 and
+   <= 16384; the product is one linear convolution per column in LDS. The
+   shifted values are rounded to complex64 (the reference stores them into the
+   complex64 array it read from the SLC file) and, when flat is given,
+   multiplied by flat[r] = exp(i k0 b_xti (inc(r) - inc0)), in FP64. */
+typedef struct osar_coreg_desc {
+  int32_t n_pol;            /* planes of the channel */
+  int32_t n_az, n_rg;       /* SLC lines and range samples */
+  int32_t n_fft;            /* M */
+  int64_t in_pol_stride;    /* complex64 elements between input planes */
+  const osar_c32* in;       /* [n_pol][n_az][n_rg] SLC as read from the file */
+  const double* kernel;     /* [M] complex128, interleaved re, im */
+  const double* flat;       /* [n_rg] complex128 flat-earth phasor, or NULL */
+  double* out;              /* out: [n_pol][n_az][n_rg] complex128 */
+} osar_coreg_desc;
+
+int osar_insar_coregister(osar_ctx* ctx, const osar_coreg_desc* d, void* stream);
+
+/* Multilook of the coregistered images (insar_processor.py:192-214 with
+   oceansar/utils/misc.py:108-197): for complex images the intensities
+   |s_c|^2 and the interferograms s_c2 conj(s_c1) of every pair c1 < c2 (the
+   reference's loop order), each smoothed along range then azimuth, cropped to
+   [az_lo, az_hi) x [rg_lo, rg_hi), and the coherence t / sqrt(I_c1 I_c2) as
+   magnitude and phase; real images (is_complex = 0) are smoothed as they are
+   into `intensity`. A flat window (rg_flat / az_flat = 1) is the reference's
+   edge-normalised moving average over offsets -(L/2) .. L - L/2 - 1, summed
+   in that order in FP64; otherwise the L normalised taps (numpy's window /
+   its sum) with zeros beyond the edges, as fftconvolve 'same'. */
+#define OSAR_ML_MAX_IMG 16
+typedef struct osar_multilook_desc {
+  int32_t n_img;            /* images (channels x polarisations), 1 .. 16 */
+  int32_t n_az, n_rg;       /* image lines and samples */
+  int32_t is_complex;       /* 1: complex128 images, 0: float64 images */
+  int32_t rg_len, az_len;   /* window lengths */
+  int32_t rg_flat, az_flat; /* 1: flat window, 0: taps */
+  int32_t az_lo, az_hi, rg_lo, rg_hi;   /* crop of the outputs */
+  int64_t img_stride;       /* elements between images */
+  const void* img;          /* [n_img][n_az][n_rg] */
+  const double* rg_taps;    /* [rg_len] when !rg_flat */
+  const double* az_taps;    /* [az_len] when !az_flat */
+  double* work;             /* osar_multilook_work_bytes() of device memory */
+  double* intensity;        /* out: [n_img][az_hi - az_lo][rg_hi - rg_lo] */
+  double* coherence;        /* out: [pairs][..] |coherence| (complex images) */
+  double* phase;            /* out: [pairs][..] its phase in (-pi, pi] */
+} osar_multilook_desc;
+
+int64_t osar_multilook_work_bytes(const osar_multilook_desc* d);
+int osar_insar_multilook(osar_ctx* ctx, const osar_multilook_desc* d, void* stream);
 
 #ifdef __cplusplus
 }

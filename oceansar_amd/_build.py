@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'liboceansar_hip.so')
-SOURCES = ['ctx.hip', 'echo.hip', 'bragg.hip', 'synth.hip', 'focus.hip', 'init.hip']
+SOURCES = ['ctx.hip', 'echo.hip', 'bragg.hip', 'synth.hip', 'focus.hip', 'init.hip', 'insar.hip']
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
          '-Wno-unused-command-line-argument']

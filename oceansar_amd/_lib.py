@@ -92,6 +92,21 @@ class FocusDesc(ctypes.Structure):
         [(n, c_void_p) for n in ('raw', 'fr', 'rcmc', 'h', 'work', 'slc')]
 
 
+class CoregDesc(ctypes.Structure):
+    _fields_ = [(n, c_int32) for n in ('n_pol', 'n_az', 'n_rg', 'n_fft')] + \
+        [('in_pol_stride', c_int64)] + \
+        [(n, c_void_p) for n in ('in_', 'kernel', 'flat', 'out')]
+
+
+class MultilookDesc(ctypes.Structure):
+    _fields_ = [(n, c_int32) for n in ('n_img', 'n_az', 'n_rg', 'is_complex', 'rg_len',
+                                       'az_len', 'rg_flat', 'az_flat', 'az_lo', 'az_hi',
+                                       'rg_lo', 'rg_hi')] + \
+        [('img_stride', c_int64)] + \
+        [(n, c_void_p) for n in ('img', 'rg_taps', 'az_taps', 'work', 'intensity', 'coherence',
+                                 'phase')]
+
+
 FIELD_ORDER = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy', 'Diffxx', 'Diffyy', 'Diffxy',
                'Vx', 'Vy', 'Vz', 'Ax', 'Ay', 'Az', 'hMTF']
 FIELD_BITS = {n: 1 << i for i, n in enumerate(FIELD_ORDER)}
@@ -102,7 +117,8 @@ EXPORTS = ['osar_ctx_create', 'osar_ctx_destroy', 'osar_last_error', 'osar_versi
            'osar_echo_deposit', 'osar_facet_bragg', 'osar_fill_zero_f64',
            'osar_surface_workspace_bytes', 'osar_surface_synth', 'osar_surface_synth_noalloc',
            'osar_sar_focus', 'osar_surface_init', 'osar_echo_batch_plan_bytes',
-           'osar_echo_accumulate_batch']
+           'osar_echo_accumulate_batch', 'osar_insar_coregister', 'osar_multilook_work_bytes',
+           'osar_insar_multilook']
 SYNTH_SCRATCH_BYTES = 256 << 20    # OSAR_SYNTH_SCRATCH_BYTES
 
 _lib = None
@@ -139,8 +155,12 @@ def load():
     lib.osar_echo_batch_plan_bytes.argtypes = [c_int32]
     lib.osar_echo_batch_plan_bytes.restype = c_int64
     lib.osar_echo_accumulate_batch.argtypes = [c_void_p, POINTER(EchoBatchDesc), c_void_p]
+    lib.osar_insar_coregister.argtypes = [c_void_p, POINTER(CoregDesc), c_void_p]
+    lib.osar_multilook_work_bytes.argtypes = [POINTER(MultilookDesc)]
+    lib.osar_multilook_work_bytes.restype = c_int64
+    lib.osar_insar_multilook.argtypes = [c_void_p, POINTER(MultilookDesc), c_void_p]
     for name, cls in ((0, EchoDesc), (1, BraggDesc), (2, SurfaceDesc), (3, FocusDesc),
-                      (4, InitDesc)):
+                      (4, InitDesc), (5, CoregDesc), (6, MultilookDesc)):
         n = lib.osar_struct_size(name)
         if n != ctypes.sizeof(cls):
             raise ImportError('descriptor %s: C size %d != ctypes size %d'

@@ -19,9 +19,12 @@ ocean_file=ocean.nc
 ocean_reuse=False
 errors_file=errors.nc
 errors_reuse=False
-proc_run=False
-insar_run=False
-ati_run=False
+proc_file=proc_data.nc
+insar_file=insar_l1b.nc
+ati_file=ati_data.txt
+proc_run={proc_run}
+insar_run={insar_run}
+ati_run={ati_run}
 L2_wavespectrum_run=False
 
 [sar]
@@ -95,6 +98,37 @@ plot_rcmc_dopp=False
 plot_rcmc_time=False
 plot_image_valid=False
 
+[insar]
+rg_ml={rg_ml}
+az_ml={az_ml}
+ml_win={ml_win}
+flatten={flatten}
+plot_save=False
+plot_path=insar_plots/
+plot_format=png
+plot_tex=False
+plot_surface=False
+plot_proc_ampl=False
+plot_coh=False
+plot_coh_all=False
+
+[ati]
+rg_ml=8
+az_ml=2
+ml_win={ml_win}
+plot_save=False
+plot_path=ati_plots/
+plot_format=png
+plot_tex=False
+plot_surface=False
+plot_proc_ampl=False
+plot_coh=False
+plot_coh_all=False
+plot_ati_phase=False
+plot_ati_phase_all=False
+plot_vel_hist=False
+plot_vel=False
+
 [srg]
 use_hmtf={hmtf}
 nesz=-30.
@@ -131,7 +165,8 @@ def cfg_text(L, dx, n_pulses, K, U, wind_dir=0., num_ch=1, b_ati=None, pol='vv',
              spread='elfouhaily', specular=False, swell=False, seed=1,
              f0=5.405e9, inc=35., prf=1800., rg_bw=100e6, alt=514e3, v=7600.,
              swell_wl=200., Hs=2., swell_dir=0., choppy=False, dt=None, b_xti=None,
-             frozen=False, hmtf=False):
+             frozen=False, hmtf=False, stages=(), rg_ml=5, az_ml=5, ml_win='hanning',
+             flatten=True):
     antL = antenna_length_for(n_pulses, prf, v, L, inc, alt, f0)
     b_ati = [0.] * num_ch if b_ati is None else list(b_ati)[:num_ch]
     freq_r = np.sqrt(const.g * 2 * np.pi / swell_wl) / (2 * np.pi)
@@ -143,7 +178,9 @@ def cfg_text(L, dx, n_pulses, K, U, wind_dir=0., num_ch=1, b_ati=None, pol='vv',
         b_xti=_arr([0.] * num_ch if b_xti is None else list(b_xti)[:num_ch]),
         npulses=n_pulses, L=L, dx=dx, dt=repr(float(dt)), K=K, wdir=wind_dir, U=U,
         swell=swell, freq_r=repr(float(freq_r)), swell_dir=swell_dir, Hs=Hs,
-        spread=spread, choppy=choppy, specular=specular, frozen=frozen, hmtf=hmtf)
+        spread=spread, choppy=choppy, specular=specular, frozen=frozen, hmtf=hmtf,
+        proc_run='proc' in stages, insar_run='insar' in stages, ati_run='ati' in stages,
+        rg_ml=rg_ml, az_ml=az_ml, ml_win=ml_win, flatten=flatten)
 
 
 WORKLOADS = ('small', 'ati_swell', 'large')

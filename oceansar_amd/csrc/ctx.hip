@@ -42,6 +42,8 @@ extern "C" int64_t osar_struct_size(int which) {
     case 2: return (int64_t)sizeof(osar_surface_desc);
     case 3: return (int64_t)sizeof(osar_focus_desc);
     case 4: return (int64_t)sizeof(osar_init_desc);
+    case 5: return (int64_t)sizeof(osar_coreg_desc);
+    case 6: return (int64_t)sizeof(osar_multilook_desc);
     default: return -1;
   }
 }

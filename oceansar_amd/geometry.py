@@ -66,3 +66,22 @@ def gr_to_geo(ground_range, orbit_alt, r_planet=const.r_earth):
     theta_i_i = spl(gr, theta_i, k=2)(ground_range)
     sr_i = spl(gr, sr, k=2)(ground_range)
     return sr_i, theta_i_i, theta_l_i
+
+
+def sr_to_geo(slant_range, orbit_alt, r_planet=const.r_earth, m_planet=const.m_earth):
+    """Ground range, incidence angle, look angle and the centripetal term
+    v_orb^2 r_track / (r_planet + h) at slant ranges (geometry.py:101-133):
+    a 2500-point look-angle table from nadir to the horizon and scipy's
+    quadratic InterpolatedUnivariateSpline over slant range, as the reference."""
+    theta_l = np.linspace(0, max_look_angle(orbit_alt, r_planet=r_planet), 2500)
+    theta_i = look_to_inc(theta_l, orbit_alt, r_planet=r_planet)
+    dth = theta_i - theta_l
+    r_track = np.cos(dth) * r_planet
+    v_orb = orbit_to_vel(orbit_alt, r_planet=r_planet, m_planet=m_planet)
+    b = v_orb ** 2 * r_track / (r_planet + orbit_alt)
+    gr = r_planet * dth
+    sr = np.sqrt((orbit_alt + r_planet - r_planet * np.cos(dth)) ** 2 +
+                 (r_planet * np.sin(dth)) ** 2)
+    spl = interpolate.InterpolatedUnivariateSpline
+    return (spl(sr, gr, k=2)(slant_range), spl(sr, theta_i, k=2)(slant_range),
+            spl(sr, theta_l, k=2)(slant_range), spl(sr, b, k=2)(slant_range))

@@ -1,9 +1,11 @@
-"""NetCDF files of the raw-data path: RawFile, OceanStateFile and ProcFile.
+"""NetCDF files of the raw-data path and its consumers: RawFile,
+OceanStateFile, ProcFile and L1bFile.
 
 Same variable names, dimensions, units and the '_r'/'_i' convention for
 complex values as oceansar/ocs_io/raw.py:160-250, state.py:7-72,
-processed.py:6-77 and netcdf.py:4-45, so downstream readers
-(sar_processor.py:71-77) find what they expect. netCDF4 is not a dependency: files are written by
+processed.py:6-77, insar_l1b.py:7-90 and netcdf.py:4-45, so downstream
+readers (sar_processor.py:71-77, insar_processor.py:80-94,
+ati_processor.py:78-96) find what they expect. netCDF4 is not a dependency: files are written by
 scipy.io.netcdf_file as NetCDF-3 64-bit-offset, which netCDF4.Dataset reads.
 NetCDF-3 has no 'u4', so the state file stores Nx/Ny as 'i4'.
 """
@@ -138,3 +140,41 @@ class ProcFile(_Handler):
                 _var(f, n, 'f8', (), u)
             _var(f, 'b_ati', 'f8', ('ch_dim',), '[m]')
             _var(f, 'b_xti', 'f8', ('ch_dim',), '[m]')
+
+
+class L1bFile(_Handler):
+    """InSAR L1b file (oceansar/ocs_io/insar_l1b.py:7-90): ml_intensity f8
+    [ch, pol, az, rg], ml_coherence / ml_phase f8 [covel, az, rg] with covel =
+    C (C - 1) / 2 pairs of the C = ch * pol images, coh_lut i4 [ch, pol, ch,
+    pol], the radar scalars, the baselines and the multilook window sizes."""
+
+    def __init__(self, file_name, mode, data_dim=None):
+        self._f = netcdf_file(file_name, mode, version=2, mmap=False)
+        if mode == 'w':
+            f = self._f
+            f.description = 'OCEANSAR InSAR L1b data file'
+            f.history = 'Created ' + time.ctime(time.time())
+            f.source = 'OCEANSAR Simulator (oceansar_amd)'
+            if not data_dim:
+                raise ValueError('Processed raw data dimensions are needed when creating a new file!')
+            if len(data_dim) != 4:
+                raise ValueError('data_dim must be (ch, pol, az, rg)')
+            for n, s in zip(('ch_dim', 'pol_dim', 'az_dim', 'rg_dim'), data_dim):
+                f.createDimension(n, int(s))
+            tot = int(data_dim[0]) * int(data_dim[1])
+            f.createDimension('covel_dim', tot * (tot - 1) // 2)
+            _var(f, 'num_ch', 'i4', (), '[]')
+            _var(f, 'ml_intensity', 'f8', ('ch_dim', 'pol_dim', 'az_dim', 'rg_dim'), 'm2/m2')
+            _var(f, 'ml_coherence', 'f8', ('covel_dim', 'az_dim', 'rg_dim'), '[]')
+            _var(f, 'ml_phase', 'f8', ('covel_dim', 'az_dim', 'rg_dim'), 'rad')
+            lut = _var(f, 'coh_lut', 'i4', ('ch_dim', 'pol_dim', 'ch_dim', 'pol_dim'), '[]')
+            lut.description = ('LUT that indicates where to find coh and phase for any '
+                               'combination in ml_coh and ml_phase')
+            for n, u in (('inc_angle', '[deg]'), ('f0', '[Hz]'), ('az_sampling', '[Hz]'),
+                         ('v_ground', '[m/s]'), ('orbit_alt', '[m]'), ('sr0', '[m]'),
+                         ('rg_sampling', '[Hz]')):
+                _var(f, n, 'f8', (), u)
+            _var(f, 'b_ati', 'f8', ('ch_dim',), '[m]')
+            _var(f, 'b_xti', 'f8', ('ch_dim',), '[m]')
+            _var(f, 'rg_ml', 'i4', (), '[]')
+            _var(f, 'az_ml', 'i4', (), '[]')

@@ -7,9 +7,11 @@ then the raw cube is trimmed to rg_samp_orig, calibrated and given complex
 noise (sar_raw_nompi.py:552-574; the noise is drawn from the seeded noise
 stream) and written with the reference's RawFile layout.
 
-sarsim mirrors oceansar/radarsim/oceansar_sarsim.py:29-83 for the raw stage.
-The processing stages after it (focusing, InSAR, ATI, L2) are not part of
-this package; their flags are reported and skipped.
+sarsim mirrors oceansar/radarsim/oceansar_sarsim.py:29-83: the raw stage,
+then, as the cfg's [sim] flags ask, SAR focusing (sar_processor.sar_focus),
+InSAR L1b processing (insar_processor.insar_process) and ATI analysis
+(ati_processor.ati_process). The L2 wave-spectrum stage is not built; its
+flag is reported and skipped.
 """
 import os
 import time
@@ -128,10 +130,22 @@ def sarsim(cfg_file, device=0):
                       os.path.join(path, sim.ocean_file), getattr(sim, 'ocean_reuse', False),
                       os.path.join(path, getattr(sim, 'errors_file', 'errors.nc')),
                       getattr(sim, 'errors_reuse', False), device=device)
-    for stage in ('proc_run', 'insar_run', 'ati_run', 'L2_wavespectrum_run'):
-        if getattr(sim, stage, False):
-            print('sarsim: %s is a downstream processing stage, not part of this package; '
-                  'skipped' % stage)
+    if getattr(sim, 'proc_run', False):
+        from .sar_processor import sar_focus
+        sar_focus(cfg.cfg_file_name, os.path.join(path, sim.raw_file),
+                  os.path.join(path, sim.proc_file), device=device)
+    if getattr(sim, 'insar_run', False):
+        from .insar_processor import insar_process
+        insar_process(cfg.cfg_file_name, os.path.join(path, sim.proc_file),
+                      os.path.join(path, sim.ocean_file), os.path.join(path, sim.insar_file),
+                      device=device)
+    if getattr(sim, 'ati_run', False):
+        from .ati_processor import ati_process
+        ati_process(cfg.cfg_file_name, os.path.join(path, sim.insar_file),
+                    os.path.join(path, sim.ocean_file), os.path.join(path, sim.ati_file),
+                    device=device)
+    if getattr(sim, 'L2_wavespectrum_run', False):
+        print('sarsim: L2_wavespectrum_run is a downstream stage not built here; skipped')
     return out
 
 

@@ -52,4 +52,13 @@ Contents and how each is pinned (see tests/test_oracle_*.py):
                           the reference's horizontal term Dx sin(inc)/sr0 that no
                           closed form fixes) is pinned bit for bit by golden
                           vectors of the reference's own two statements.
+  insar.coregister / l1b / ati_stats  (InSAR / ATI processors) the azimuth
+                          shift pinned by a roll for integer shifts and by the
+                          tone eigenvalue at the reference's rolled frequency
+                          index (the half-bin offset of odd lengths included);
+                          smooth1d / smooth by brute-force window sums; the
+                          coherence by identical and phase-offset images;
+                          uwphase by the left-edge centre of the fullest bin;
+                          sr_to_geo (host) bit for bit by golden vectors of
+                          the reference's own source text.
 """

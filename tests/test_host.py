@@ -112,7 +112,8 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
     assert set(names) == set(_lib.EXPORTS)
     assert lib.osar_version() >= 100
-    for i, cls in enumerate((_lib.EchoDesc, _lib.BraggDesc, _lib.SurfaceDesc, _lib.FocusDesc)):
+    for i, cls in enumerate((_lib.EchoDesc, _lib.BraggDesc, _lib.SurfaceDesc, _lib.FocusDesc,
+                             _lib.InitDesc, _lib.CoregDesc, _lib.MultilookDesc)):
         assert lib.osar_struct_size(i) == ctypes.sizeof(cls)
 
 
