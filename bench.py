@@ -285,6 +285,102 @@ def bench_serve(args):
     }
 
 
+def bench_insar(args):
+    """InSAR L1b processing at the large workload's SLC size: 4 channels x 7555
+    lines x 624 range samples (the valid lines of the focused 8192-pulse raw
+    cube), complex64 planes resident on the device. A step is what
+    insar_process does on the device: the azimuth shift and flattening of every
+    channel (osar_insar_coregister) and the 5 x 5 Hann multilook of the
+    intensities, the six interferograms and their coherences over the region
+    of interest of the 1024 m scene (osar_insar_multilook). value = SLC
+    samples (channel x line x range sample) per second."""
+    import torch
+    from oceansar_amd import _lib
+    from oceansar_amd.insar_processor import CoregPlan, coregister_channel, multilook
+    nch, naz, nrg = 4, 7555, 624
+    rng = np.random.default_rng(9)
+    x = (rng.standard_normal((nch, 1, naz, nrg), dtype=np.float32) +
+         1j * rng.standard_normal((nch, 1, naz, nrg), dtype=np.float32)).astype(np.complex64)
+    dev = torch.device('cuda', 0)
+    slc = torch.as_tensor(x, device=dev)
+    ctx = _lib.context(0)
+    v, prf = 6900., 1800.
+    # the host tables of each channel (kernel spectrum, flat-earth phasor), built once
+    plans = [CoregPlan(naz, -b / (v / prf), np.exp(1j * 1e-3 * b * np.arange(nrg)), dev)
+             for b in (0., 5., 10., 20.)]
+    crop = (naz // 2 - 133, naz // 2 + 134, 0, 431)
+    images = torch.empty((nch, naz, nrg), dtype=torch.complex128, device=dev)
+    ev = []
+
+    def step(record):
+        for ch in range(nch):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            coregister_channel(ctx, slc[ch], plans[ch], out=images[ch:ch + 1])
+            b.record()
+            if record:
+                ev.append(('coregister', a, b))
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        multilook(ctx, images, 5, 5, 'hanning', crop)
+        b.record()
+        if record:
+            ev.append(('multilook', a, b))
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    progress('insar steps done: %.3f ms per step' % (dt * 1e3))
+    per = {}
+    co = []
+    for k, a, b in ev:
+        ms = a.elapsed_time(b)
+        per[k] = per.get(k, 0.) + ms / args.steps
+        if k == 'coregister':
+            co.append(ms)
+    samples = nch * naz * nrg
+    co_bytes = naz * nrg * (8 + 16)          # complex64 in, complex128 out per channel launch
+    cpu = None
+    if not args.no_cpu:
+        # the numpy restatement of the same step (oracle/insar.py), on the host cores
+        import oracle.insar as oi
+        threads, n_os, model = host_cores()
+        t0 = time.perf_counter()
+        p = oi.coregister(x.copy(), [0., 5., 10., 20.], v, prf)
+        p = p * np.stack([np.exp(1j * 1e-3 * b * np.arange(nrg)) for b in (0., 5., 10., 20.)]
+                         )[:, None, None, :]
+        oi.l1b(p, 5, 5, 'hanning', *crop)
+        t_cpu = time.perf_counter() - t0
+        cpu = {'value': samples / t_cpu, 'unit': 'samples/s', 'cores': 1, 'kind': 'port',
+               'sample': 'the whole step (4 x 7555 x 624), numpy complex128 FFTs and scipy '
+                         'fftconvolve as the reference; %.1f s' % t_cpu,
+               'host': {'os_cpu_count': n_os, 'cpu_model': model}}
+        progress('insar cpu baseline done (%.1f s)' % t_cpu)
+    achieved = co_bytes / (np.mean(co) * 1e-3) / 1e9
+    return {
+        'metric': 'SLC samples/s (InSAR L1b processing: azimuth shift, flattening, multilook, '
+                  'coherence)',
+        'value': samples / dt, 'unit': 'samples/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': dt * 1e3, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64 (complex64 SLC in)',
+        'data': 'synthetic (seeded complex64 SLC planes)',
+        'config': {'workload': 'InSAR L1b of the large workload: 4 ch x 7555 lines x 624 samples, '
+                               '5x5 Hann multilook over 267 x 431', 'stage_ms_per_step': per},
+        'roofline': {'bound': 'hbm', 'kernel': 'coreg_kernel', 'achieved': achieved,
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': None, 'avg_launch_ms': float(np.mean(co)),
+                     'algorithmic_bytes_per_launch': co_bytes,
+                     'note': 'per channel launch: 8 B complex64 read + 16 B complex128 written '
+                             'per SLC sample; four 8192-point FP64 FFTs per column in LDS'},
+        'cpu_baseline': cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -310,10 +406,11 @@ def main():
                     help='two streams: synthesis/Bragg of chunk c+1 beside the accumulation '
                          'of chunk c (measured no faster on the large workload: every stage '
                          'already fills the GPU)')
-    ap.add_argument('--form', default='plain', choices=('plain', 'batch', 'serve'),
+    ap.add_argument('--form', default='plain', choices=('plain', 'batch', 'serve', 'insar'),
                     help='plain: one scene (the headline line); batch: 64 mixed scenes, '
                          'plain vs concurrent vs one graph launch; serve: Poisson job '
-                         'arrivals, p50/p99 latency vs sequential plain execution')
+                         'arrivals, p50/p99 latency vs sequential plain execution; insar: '
+                         'InSAR L1b processing of the large workload\'s 4-channel SLC')
     ap.add_argument('--jobs', type=int, default=None, help='scenes (batch) or jobs (serve)')
     ap.add_argument('--streams', type=int, default=4)
     ap.add_argument('--host-init', action='store_true',
@@ -332,6 +429,9 @@ def main():
         return
     if args.form == 'serve':
         print(json.dumps(bench_serve(args)))
+        return
+    if args.form == 'insar':
+        print(json.dumps(bench_insar(args)))
         return
 
     import torch

@@ -81,38 +81,48 @@ def flat_earth_phasor(rg_size, sr0, rg_sampling, alt, k0, b_xti, inc_angle):
     return np.exp(1j * k0 * b_xti * (inc - inc_angle))
 
 
-def coregister_channel(ctx, slc, shift_dist, flat=None, out=None):
+class CoregPlan:
+    """Device tables of one channel's shift: the kernel spectrum for n_az
+    lines and shift_dist lines, and the flat-earth phasor (None: none)."""
+
+    def __init__(self, n_az, shift_dist, flat=None, device=0):
+        import torch
+        dev = torch.device('cuda', device) if isinstance(device, int) else device
+        self.n_az = int(n_az)
+        self.n_fft, G = shift_kernel(self.n_az, shift_dist)
+        self.kernel = torch.as_tensor(G, dtype=torch.complex128, device=dev)
+        self.flat = None if flat is None else torch.as_tensor(
+            np.asarray(flat, dtype=np.complex128), device=dev)
+
+
+def coregister_channel(ctx, slc, plan, out=None):
     """Shift one channel's planes slc (complex64 device tensor [pol][az][rg])
-    by shift_dist lines along azimuth and multiply by the flat-earth phasor
-    flat ([rg] complex128, host or device; None: no flattening). Returns the
-    complex128 device tensor [pol][az][rg] (out, when given)."""
+    along azimuth and flatten them as the CoregPlan says, on the current
+    stream. Returns the complex128 device tensor [pol][az][rg] (out, when
+    given)."""
     import torch
     if slc.dtype != torch.complex64 or slc.dim() != 3:
         raise TypeError('slc must be a complex64 [pol][az][rg] device tensor')
     npol, naz, nrg = slc.shape
     if slc.stride(2) != 1 or slc.stride(1) != nrg:
         raise TypeError('slc lines must be contiguous')
-    dev = slc.device
-    M, G = shift_kernel(naz, shift_dist)
-    G_d = torch.as_tensor(G, dtype=torch.complex128, device=dev)
-    flat_d = None if flat is None else torch.as_tensor(np.asarray(flat, dtype=np.complex128)
-                                                       if not torch.is_tensor(flat) else flat,
-                                                       dtype=torch.complex128, device=dev)
-    if flat_d is not None and flat_d.numel() != nrg:
-        raise ValueError('flat phasor of %d samples for %d range samples' % (flat_d.numel(), nrg))
+    if naz != plan.n_az:
+        raise ValueError('plan for %d lines, slc has %d' % (plan.n_az, naz))
+    if plan.flat is not None and plan.flat.numel() != nrg:
+        raise ValueError('flat phasor of %d samples for %d range samples'
+                         % (plan.flat.numel(), nrg))
     if out is None:
-        out = torch.empty((npol, naz, nrg), dtype=torch.complex128, device=dev)
+        out = torch.empty((npol, naz, nrg), dtype=torch.complex128, device=slc.device)
     elif out.dtype != torch.complex128 or tuple(out.shape) != (npol, naz, nrg) \
             or not out.is_contiguous():
         raise TypeError('out must be a contiguous complex128 [pol][az][rg] tensor')
     d = _lib.CoregDesc()
-    d.n_pol, d.n_az, d.n_rg, d.n_fft = npol, naz, nrg, M
+    d.n_pol, d.n_az, d.n_rg, d.n_fft = npol, naz, nrg, plan.n_fft
     d.in_pol_stride = slc.stride(0)
-    d.in_, d.kernel = slc.data_ptr(), G_d.data_ptr()
-    d.flat = None if flat_d is None else flat_d.data_ptr()
+    d.in_, d.kernel = slc.data_ptr(), plan.kernel.data_ptr()
+    d.flat = None if plan.flat is None else plan.flat.data_ptr()
     d.out = out.data_ptr()
     _lib.check(ctx.lib.osar_insar_coregister(ctx.h, ctypes.byref(d), _lib.stream_ptr()))
-    torch.cuda.current_stream(dev).synchronize()     # G_d and flat_d must outlive the launch
     return out
 
 
@@ -160,8 +170,9 @@ def multilook(ctx, images, rg_ml, az_ml, window='flat', crop=None):
     d.img, d.work, d.intensity = images.data_ptr(), work.data_ptr(), inten.data_ptr()
     if pairs:
         d.coherence, d.phase = coh.data_ptr(), pha.data_ptr()
+    # taps and workspace are freed to torch's stream-ordered cache: later work
+    # that reuses them is queued behind these launches
     _lib.check(ctx.lib.osar_insar_multilook(ctx.h, ctypes.byref(d), _lib.stream_ptr()))
-    torch.cuda.current_stream(dev).synchronize()
     del keep, work
     return inten, coh, pha
 
@@ -234,8 +245,8 @@ def insar_process(cfg_file, proc_output_file, ocean_file, output_file, device=0)
     for ch in range(nch):
         flat = (flat_earth_phasor(rg_size, sr0, rg_sampling, alt, k0, b_xti[ch], inc_angle)
                 if flatten else None)
-        coregister_channel(ctx, slc[ch], -inter_chan_shift_dist[ch], flat,
-                           out=images[ch * npol:(ch + 1) * npol])
+        plan = CoregPlan(az_size, -inter_chan_shift_dist[ch], flat, dev)
+        coregister_channel(ctx, slc[ch], plan, out=images[ch * npol:(ch + 1) * npol])
     del slc
     inten, coh, pha = multilook(ctx, images, rg_ml, az_ml, ml_win, (a0, a1, r0, r1))
     del images

@@ -35,14 +35,14 @@ def test_coregister_matches_oracle(need_gpu, naz, nrg, npol, flatten):
     (M <= 8192), the two-half form beyond (M = 16384, up to 8192 lines); 7555
     lines x 624 samples is the large workload's SLC."""
     import torch
-    from oceansar_amd.insar_processor import coregister_channel
+    from oceansar_amd.insar_processor import CoregPlan, coregister_channel
     rng = np.random.default_rng(naz + nrg)
     x = (rng.standard_normal((1, npol, naz, nrg)) +
          1j * rng.standard_normal((1, npol, naz, nrg))).astype(np.complex64)
     b, v, prf = 7.5, 6900., 1800.
     flat = np.exp(1j * rng.uniform(-3, 3, nrg)) if flatten else None
     dev = torch.as_tensor(x[0], device='cuda')
-    got = coregister_channel(_ctx(), dev, -b / (v / prf), flat).cpu().numpy()
+    got = coregister_channel(_ctx(), dev, CoregPlan(naz, -b / (v / prf), flat)).cpu().numpy()
     want = oi.coregister(x.copy(), [b], v, prf)[0].astype(np.complex128)
     if flatten:
         want = want * flat[None, None, :]
