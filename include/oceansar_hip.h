@@ -124,7 +124,8 @@ typedef struct osar_echo_desc {
   /* Bragg amplitudes sqrt(RCS), float32 [n_rcs_slots][n_pol][2][nx][ny];
      component 0 pairs with scat_m, component 1 with scat_p */
   const float* amp;
-  const int32_t* rcs_slot;  /* [n_pulses_total] RCS refresh index in force */
+  const int32_t* rcs_slot;  /* [n_pulses_total] RCS refresh index in force:
+                               non-decreasing, at most +1 from one pulse to the next */
   /* scatterer series, extended: s_ext[i] = s[i mod N], i < N + n_pulses_total */
   const osar_c32* scat_m;
   const osar_c32* scat_p;

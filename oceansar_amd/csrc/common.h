@@ -79,6 +79,15 @@ __device__ __forceinline__ float2 cis_fp64(double phase) {
   return make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
 }
 
+// exp(2 pi i r) for an FP64 phase r already in revolutions: its fractional part
+// (v_fract_f64, exact) is rounded to float32 and goes through the hardware
+// sin/cos, which take revolutions and reduce their argument themselves; half
+// the FP64 work of cis_fp64 when the caller can form the phase in revolutions.
+__device__ __forceinline__ float2 cis_rev(double r) {
+  const float f = (float)__builtin_amdgcn_fract(r);
+  return make_float2(__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f));
+}
+
 // a / b for b > 0 given inv = 1 / b correctly rounded: one residual correction
 // of a * inv gives the correctly rounded quotient (Markstein), i.e. the same
 // bits as IEEE division, at three FP64 operations.

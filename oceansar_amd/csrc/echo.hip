@@ -226,7 +226,8 @@ struct FacetBases {
   const float* dy;
   const float* dz;
   const float* s[5];
-  const float* amp[NPOL][2];   // per lane
+  const float* amp[NPOL][2];   // planes of the group's first refresh slot
+  uint32_t aoff;               // per lane: byte offset of the lane's slot from them
 };
 
 // Per-lane inputs of one facet (one column x, the lane's row and pulse),
@@ -267,10 +268,11 @@ __device__ __forceinline__ void load_scat(const osar_echo_desc& d, const FacetBa
                                           bool bragg, const FacetIdx& fi,
                                           FacetIn<NPOL, SPEC>& f) {
   if (bragg) {
+    const uint32_t oa = idx4 + B.aoff;
 #pragma unroll
     for (int pol = 0; pol < NPOL; ++pol) {
-      f.amp[pol][0] = load_at(B.amp[pol][0], idx4);
-      f.amp[pol][1] = load_at(B.amp[pol][1], idx4);
+      f.amp[pol][0] = load_at(B.amp[pol][0], oa);
+      f.amp[pol][1] = load_at(B.amp[pol][1], oa);
     }
     // byte offsets below 2^32: the host bounds the series to 2^29 entries
     f.sm = load_at(d.scat_m, (uint32_t)(fi.m + p) << 3);
@@ -428,7 +430,16 @@ __device__ __forceinline__ void echo_item(const osar_echo_desc& d, unsigned char
   const bool bragg = (d.flags & OSAR_ECHO_BRAGG) != 0;
   int rslot = bragg ? d.rcs_slot[pc] - d.rcs0 : 0;
   const bool bragg_ok = bragg && rslot >= 0 && rslot < d.n_rcs_slots;
-  if (!bragg_ok) rslot = 0;
+  // Amplitude planes by 32-bit byte offsets from the refresh slot in force at
+  // the group's first pulse (wave-uniform): the slot index does not decrease
+  // with the pulse and grows by at most one per pulse, so a lane is 0 .. P - 1
+  // slots above it (clamped, so that any slot table stays in bounds; the host
+  // bounds P slots of planes to 32 bits). Lanes without a refresh in the
+  // call's slots read the group's slot and do not use the values.
+  const int rs0 = bragg ? min(max(d.rcs_slot[pstart] - d.rcs0, 0), max(d.n_rcs_slots - 1, 0)) : 0;
+  if (!bragg_ok) rslot = rs0;
+  const uint32_t aoff =
+      (uint32_t)(min(max(rslot - rs0, 0), P - 1) * NPOL * 2) * (uint32_t)(plane * 4);
   // Snapshot fields from the group's first snapshot m_lo = pstart / K (every
   // lane's m is at most P / K + 1 above it; the host bounds the byte offsets to
   // 32 bits); at a snapshot pulse (rk = 0) the second value repeats the first.
@@ -447,10 +458,11 @@ __device__ __forceinline__ void echo_item(const osar_echo_desc& d, unsigned char
     }
 #pragma unroll
     for (int pol = 0; pol < NPOL; ++pol) {
-      const float* a0 = d.amp + ((size_t)(rslot * NPOL + pol) * 2) * plane;
+      const float* a0 = d.amp + ((size_t)(rs0 * NPOL + pol) * 2) * plane;
       B.amp[pol][0] = a0;
       B.amp[pol][1] = a0 + plane;
     }
+    B.aoff = aoff;
   }
   const uint32_t f0 = (uint32_t)((m - pstart / d.K) * plane * 4);
   const uint32_t f1 = f0 + (rk != 0 ? (uint32_t)(plane * 4) : 0u);
@@ -463,7 +475,9 @@ __device__ __forceinline__ void echo_item(const osar_echo_desc& d, unsigned char
   L.nbins = d.nbins;
   L.npt = d.n_pulses_total;
   L.raw = d.raw;
-  const double two_k0 = 2 * d.k0;
+  // Bragg phases in revolutions: -2 k0 sr / (2 pi) and cap_rate (p + 1) / (2 pi)
+  const double surf_rev = -(2 * d.k0) * osar::kInvTwoPi;
+  const double cap_rev = (double)(p + 1) * osar::kInvTwoPi;
   const double inv_binsize = 1.0 / d.binsize;
   const int spol = NPOL == 2 ? 2 : (d.pol == 1 ? 1 : 0);
   double spec_acc = 0.;
@@ -527,10 +541,12 @@ __device__ __forceinline__ void echo_item(const osar_echo_desc& d, unsigned char
 #pragma unroll
       for (int pol = 0; pol < NPOL; ++pol) scene[pol] = make_float2(0.f, 0.f);
       if (bragg_ok) {
-        const double surf = -two_k0 * g.srs;
-        const double cap = uniform_load(d.cap_rate, x) * (double)(p + 1);
-        const float2 e0 = osar::cis_fp64(surf - cap);
-        const float2 e1 = osar::cis_fp64(surf + cap);
+        // exp(i (-2 k0 sr_surface -+ cap_phase)) (sar_raw_nompi.py:470-480),
+        // the FP64 phases formed in revolutions
+        const double surf = g.srs * surf_rev;
+        const double cap = uniform_load(d.cap_rate, x) * cap_rev;
+        const float2 e0 = osar::cis_rev(surf - cap);
+        const float2 e1 = osar::cis_rev(surf + cap);
         const float2 b0 = osar::cmul(e0, make_float2(cur.sm.re, cur.sm.im));
         const float2 b1 = osar::cmul(e1, make_float2(cur.sp.re, cur.sp.im));
 #pragma unroll
@@ -1027,6 +1043,12 @@ int check_echo_desc(const osar_echo_desc* d, int* P_out, int* W_out, int* R_out)
   OSAR_REQUIRE((uint64_t)d->nx * d->ny * 4 * (uint64_t)(P / d->K + 3) < (1ull << 32),
                OSAR_EUNSUPPORTED, "surface of %d x %d facets too large for 32-bit field offsets",
                d->nx, d->ny);
+  // Bragg amplitudes likewise, from the refresh slot of a group's first pulse
+  // (its lanes are at most P - 1 slots further on, n_pol x 2 planes per slot)
+  OSAR_REQUIRE(!(d->flags & OSAR_ECHO_BRAGG) ||
+                   (uint64_t)d->nx * d->ny * 4 * (uint64_t)(P * d->n_pol * 2) < (1ull << 32),
+               OSAR_EUNSUPPORTED,
+               "surface of %d x %d facets too large for 32-bit amplitude offsets", d->nx, d->ny);
   return OSAR_OK;
 }
 
