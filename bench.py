@@ -203,7 +203,7 @@ def bench_batch(args):
     torch.cuda.synchronize()
     # persistent form: per round of chunks, one persistent accumulator launch
     # over every scene (osar_echo_accumulate_batch), all rounds in one graph
-    pb = batch.PersistentBatch(scenes, n_streams=args.streams, chunk=32)
+    pb = batch.PersistentBatch(scenes, n_streams=args.streams, chunk=args.batch_chunk)
     t = time.perf_counter()
     pb.capture()
     t_pcapture = time.perf_counter() - t
@@ -412,6 +412,8 @@ def main():
                          'arrivals, p50/p99 latency vs sequential plain execution; insar: '
                          'InSAR L1b processing of the large workload\'s 4-channel SLC')
     ap.add_argument('--jobs', type=int, default=None, help='scenes (batch) or jobs (serve)')
+    ap.add_argument('--batch-chunk', type=int, default=32,
+                    help='batch form: snapshot intervals per persistent round')
     ap.add_argument('--streams', type=int, default=4)
     ap.add_argument('--host-init', action='store_true',
                     help='serve form: build the job surfaces on the host instead of the device')

@@ -13,12 +13,18 @@ SOURCES = ['ctx.hip', 'echo.hip', 'bragg.hip', 'synth.hip', 'focus.hip', 'init.h
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
          '-Wno-unused-command-line-argument']
-# Per-source flags. synth.hip: the row pass's loop over a snapshot pair would
-# have its FP64 sin/cos polynomial constants hoisted out of it by the machine
-# loop-invariant code motion and spilled to scratch (128 VGPRs, 60 B of
-# scratch); without that pass they are rematerialised where used (117 VGPRs,
-# no scratch).
-SOURCE_FLAGS = {'synth.hip': ['-mllvm', '-disable-machine-licm']}
+# Per-source flags: no machine loop-invariant code motion. That pass hoists
+# constants and addresses out of the kernels' loops and then spills them:
+# synth.hip's row pass over a snapshot pair kept its FP64 sin/cos polynomial
+# constants live across both snapshots (128 VGPRs, 60 B of scratch; without
+# the pass 117 VGPRs, no scratch); echo.hip's specular and cross-track
+# accumulators and persistent batch kernels spilled 16-228 B (none without the
+# pass: the 1024^2 specular workload's accumulation 60.1 -> 53.2 ms, the plain
+# 4-channel kernel 127 -> 103 VGPRs at the same time); bragg.hip and init.hip
+# need fewer registers without it.
+_NO_MLICM = ['-mllvm', '-disable-machine-licm']
+SOURCE_FLAGS = {'synth.hip': _NO_MLICM, 'echo.hip': _NO_MLICM, 'bragg.hip': _NO_MLICM,
+                'init.hip': _NO_MLICM}
 
 
 def build(verbose=False):
