@@ -283,8 +283,8 @@ class DeviceScene:
     def default_shape(self, lds_budget=160 * 1024, n_ch=None, pulses=(8, 4)):
         """The plain kernels are built for two workgroups per CU (4 waves per SIMD):
         the largest P in (8, 4) and the widest column chunk whose LDS lines fit
-        twice in a CU (P = 8 loads each facet once for 8 pulses), 256-row tiles;
-        otherwise (specular, XTI, or no fit) the largest P whose lines fit once. n_ch:
+        twice in a CU (P = 8 loads each facet once for 8 pulses); otherwise
+        (specular, XTI, or no fit) the largest P whose lines fit once. n_ch:
         size the lines for this many channels (a batch kernel built for the
         batch's largest channel count) instead of the scene's; pulses: the P
         values tried for two workgroups per CU."""
@@ -292,10 +292,10 @@ class DeviceScene:
         nch = sc.num_ch if n_ch is None else int(n_ch)
         if not (sc.spec_enable or np.any(sc.b_xti != 0)):
             for P in pulses:
-                # 256 rows per workgroup: a quarter of the workgroups of 64-row
-                # tiles, so a quarter of the LDS clearing and line flushes
-                # (0.8 % faster on the 2048^2 workload)
-                R = min(max(256, 8 * (64 // P)), sc.Ny)
+                # one row group per wave (64 rows at P = 8): 256-row tiles measured
+                # 0.8 % faster on the 2048^2 workload but read the surface fields
+                # 1.6x over from HBM (a larger working set per XCD's L2)
+                R = min(8 * (64 // P), sc.Ny)
                 for cols in (sc.Nx, sc.Nx // 2, sc.Nx // 4, sc.Nx // 8):
                     if cols < 64 or (cols < sc.Nx and not self.sr_monotonic):
                         break
