@@ -269,4 +269,85 @@ __device__ __forceinline__ void lds_ifft(double2* buf, int N, int logN, int nseq
   }
 }
 
+// lds_ifft with the length fixed at compile time (N = 2^LOGN >= 64, sequences
+// N apart): every pass's length, stride, twiddle step and padded offsets are
+// constants, so the loads and stores of a butterfly share one address and
+// take immediate offsets, and the pass loop is unrolled. Same arithmetic in
+// the same order as lds_ifft, so the same bits.
+template <int NT, int R, int BPT, int LOGN, int LOGNS>
+__device__ __forceinline__ void stockham_pass_n(double2* buf, int nseq, const double2* tw) {
+  constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
+  constexpr int N = 1 << LOGN, Ns = 1 << LOGNS;
+  constexpr int lnb = LOGN - LR, nb = 1 << lnb;
+  static_assert(nb % 8 == 0 && (Ns == 1 || Ns % 8 == 0), "padded strides need multiples of 8");
+  constexpr int rd_step = nb + (nb >> 3);
+  constexpr int wr_step = Ns + (Ns >> 3);
+  const int tid = threadIdx.x;
+  const int total = nseq << lnb;
+  double2 v[BPT][R];
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int id = tid + b * NT;
+    if (id < total) {
+      const int s = id >> lnb, j = id & (nb - 1);
+      const double2* p = buf + pad(s * N + j);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = p[r * rd_step];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int id = tid + b * NT;
+    if (id < total) {
+      const int s = id >> lnb, j = id & (nb - 1);
+      const int jm = j & (Ns - 1);
+      if constexpr (Ns > 1) {
+        const double2 w1 = twiddle(tw, jm << (LOGN - LOGNS - LR));
+        double2 wr = w1;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          v[b][r] = cmul2(v[b][r], wr);
+          if (r + 1 < R) wr = cmul2(wr, w1);
+        }
+      }
+      dft<R>(v[b]);
+      double2* p = buf + pad(s * N + (j - jm) * R + jm);
+#pragma unroll
+      for (int r = 0; r < R; ++r) p[r * (Ns == 1 ? 1 : wr_step)] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int NT, int EMAX, int LRMAX, int LOGN, int LOGNS = 0>
+__device__ __forceinline__ void lds_ifft_n(double2* buf, int nseq, const double2* tw) {
+  if constexpr (LOGNS < LOGN) {
+    constexpr int left = LOGN - LOGNS;
+    constexpr int lr0 = left < LRMAX ? left : LRMAX;
+    constexpr int LR = (left - lr0 == 1 && lr0 > 2) ? lr0 - 1 : lr0;
+    constexpr int R = 1 << LR;
+    stockham_pass_n<NT, R, (EMAX / R > 0 ? EMAX / R : 1), LOGN, LOGNS>(buf, nseq, tw);
+    lds_ifft_n<NT, EMAX, LRMAX, LOGN, LOGNS + LR>(buf, nseq, tw);
+  }
+}
+
+// lds_ifft of sequences N apart, taking the compile-time path for N = 512 ..
+// 8192 (as far as a workgroup holds one sequence: EMAX NT >= N).
+template <int NT, int EMAX, int LRMAX, bool SHORT, int L = 13>
+__device__ __forceinline__ void lds_ifft_any(double2* buf, int N, int logN, int nseq,
+                                             const double2* tw) {
+  if constexpr (L >= 9) {
+    if constexpr (EMAX * NT >= (1 << L)) {
+      if (logN == L) {
+        lds_ifft_n<NT, EMAX, LRMAX, L>(buf, nseq, tw);
+        return;
+      }
+    }
+    lds_ifft_any<NT, EMAX, LRMAX, SHORT, L - 1>(buf, N, logN, nseq, tw);
+  } else {
+    lds_ifft<NT, EMAX, LRMAX, SHORT>(buf, N, logN, nseq, N, tw);
+  }
+}
+
 }  // namespace

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kFocThreads) void focus_rows_kernel(FocusArgs a) {
   }
   __syncthreads();
   if (n >= 64)
-    lds_ifft<kFocThreads, kFocEmax, 3, true>(buf, n, a.lognrg, ns, n, tw);
+    lds_ifft_any<kFocThreads, kFocEmax, 3, true>(buf, n, a.lognrg, ns, tw);
   else
     lds_ifft<kFocThreads, kFocEmax, 3, false>(buf, n, a.lognrg, ns, n, tw);
   const double scale = 1.0 / (double)n;
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kFocThreads) void focus_cols_kernel(FocusArgs a) {
   }
   __syncthreads();
   if (n >= 64)
-    lds_ifft<kFocThreads, kFocEmax, 3, true>(buf, n, a.lognaz, cb, n, tw);
+    lds_ifft_any<kFocThreads, kFocEmax, 3, true>(buf, n, a.lognaz, cb, tw);
   else
     lds_ifft<kFocThreads, kFocEmax, 3, false>(buf, n, a.lognaz, cb, n, tw);
   const double scale = 1.0 / (double)n;

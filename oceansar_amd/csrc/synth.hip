@@ -456,7 +456,7 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
       __syncthreads();
       // the paired spectrum's component a stays in the buffer for ta + 1
       if (mode == 1 && ns == 2) continue;
-      lds_ifft<kRowThreads, 2 * kMaxN / kRowThreads, 3, SHORT>(S, nx, logx, nrow, nx, tw);
+      lds_ifft_any<kRowThreads, 2 * kMaxN / kRowThreads, 3, SHORT>(S, nx, logx, nrow, tw);
       // column-blocked plane W[x / cbw][y][x % cbw] of this spectrum and snapshot
       // (the paired spectrum's at its first snapshot ta)
       unsigned char* wp =
@@ -542,7 +542,7 @@ __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(Synt
     if (e < ny * cb) C[pad((e & (cb - 1)) * ny + (e >> a.lcb))] = fill[k];
   }
   __syncthreads();
-  lds_ifft<kColThreads, kColElems / kColThreads, kColRadixLog, SHORT>(C, ny, a.logy, cb, ny, tw);
+  lds_ifft_any<kColThreads, kColElems / kColThreads, kColRadixLog, SHORT>(C, ny, a.logy, cb, tw);
   const double scale = 1.0 / ((double)nx * (double)ny);
   const double t = a.times[ti];
   const size_t plane = (size_t)nx * ny;
