@@ -381,6 +381,151 @@ def bench_insar(args):
     }
 
 
+def bench_focus(args):
+    """SAR focusing of the large workload's raw cube (sar_focus's processing
+    loop on the device, osar_sar_focus): 4 channels x 8192 pulses x 624 range
+    samples, complex64 raw planes resident on the device, padded to 8192 x 1024
+    and focused in FP64 (four in-LDS FFT passes over a complex128 workspace).
+    value = raw samples (channel x pulse x range sample) focused per second."""
+    import ctypes
+    import torch
+    from oceansar_amd import _lib
+    from oceansar_amd.sar_processor import FocusParams
+    nch, naz0, nrg0 = 4, 8192, 624
+    n_az, n_rg = 8192, 1024
+    rng = np.random.default_rng(7)
+    x = (rng.standard_normal((1, nch, naz0, nrg0), dtype=np.float32) +
+         1j * rng.standard_normal((1, nch, naz0, nrg0), dtype=np.float32)).astype(np.complex64)
+    dev = torch.device('cuda', 0)
+    raw = torch.as_tensor(x, device=dev)
+    # C band, 1800 Hz PRF, 110 MHz range sampling, 1.528 m antenna (the large workload)
+    params = FocusParams(1800., 110e6, 5.405e9, 6900., 616400.5, 1100., 0.6, ant_l=1.528)
+    fr, rcmc, h = params.tables(n_az, n_rg)
+    lo, hi = params.valid_rows(naz0)
+    fr_d = torch.as_tensor(fr, dtype=torch.float64, device=dev)
+    rcmc_d = torch.as_tensor(rcmc, dtype=torch.float64, device=dev)
+    h_d = torch.as_tensor(h, dtype=torch.complex128, device=dev)
+    work = torch.empty(n_az * n_rg, dtype=torch.complex128, device=dev)
+    slc = torch.empty((nch, 1, hi - lo, nrg0), dtype=torch.complex128, device=dev)
+    ctx = _lib.context(0)
+    descs = []
+    for ch in range(nch):
+        d = _lib.FocusDesc()
+        d.n_pol, d.n_az0, d.n_rg0, d.n_az, d.n_rg = 1, naz0, nrg0, n_az, n_rg
+        d.az_lo, d.az_hi = lo, hi
+        d.raw_pol_stride = raw.stride(0)
+        d.raw = raw[0, ch].data_ptr()
+        d.fr, d.rcmc, d.h = fr_d.data_ptr(), rcmc_d.data_ptr(), h_d.data_ptr()
+        d.work, d.slc = work.data_ptr(), slc[ch].data_ptr()
+        descs.append(d)
+    ev = []
+
+    def step(record):
+        for d in descs:
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            _lib.check(ctx.lib.osar_sar_focus(ctx.h, ctypes.byref(d), _lib.stream_ptr()))
+            b.record()
+            if record:
+                ev.append((a, b))
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    progress('focus steps done: %.3f ms per step' % (dt * 1e3))
+    ms = [a.elapsed_time(b) for a, b in ev]
+    samples = nch * naz0 * nrg0
+    # per channel call: complex64 raw read, the complex128 workspace written by
+    # the range pass, read and written by the two middle passes, read by the
+    # last one, and the kept complex128 SLC written
+    nbytes = (8 * naz0 * nrg0 + 16 * n_az * n_rg * 5 + 16 * (hi - lo) * nrg0)
+    cpu = None
+    if not args.no_cpu:
+        import oracle.focus as of
+        threads, n_os, model = host_cores()
+        t0 = time.perf_counter()
+        of.focus_channel(x[:, 0], 1800., 110e6, 5.405e9, 6900., 616400.5, 1100., 0.6,
+                         None, None, ant_l=1.528)
+        t_cpu = time.perf_counter() - t0
+        cpu = {'value': naz0 * nrg0 / t_cpu, 'unit': 'samples/s', 'cores': 1, 'kind': 'port',
+               'sample': 'one channel (8192 x 624), numpy complex128 FFTs as the reference; '
+                         '%.1f s' % t_cpu, 'host': {'os_cpu_count': n_os, 'cpu_model': model}}
+        progress('focus cpu baseline done (%.1f s)' % t_cpu)
+    achieved = nbytes / (np.mean(ms) * 1e-3) / 1e9
+    return {
+        'metric': 'raw samples/s (SAR focusing: range/azimuth FFTs, RCMC, azimuth compression)',
+        'value': samples / dt, 'unit': 'samples/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': dt * 1e3, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64 (complex64 raw in)',
+        'data': 'synthetic (seeded complex64 raw planes)',
+        'config': {'workload': 'sar_focus of the large workload: 4 ch x 8192 pulses x 624 '
+                               'samples, padded to 8192 x 1024, %d lines kept' % (hi - lo)},
+        'roofline': {'bound': 'hbm', 'kernel': 'osar_sar_focus (4 passes)', 'achieved': achieved,
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': None, 'avg_launch_ms': float(np.mean(ms)),
+                     'algorithmic_bytes_per_launch': nbytes,
+                     'note': 'per channel call (four kernel launches): 8 B raw read, five '
+                             'complex128 workspace sweeps, the kept SLC written'},
+        'cpu_baseline': cpu,
+    }
+
+
+def bench_init(args):
+    """Surface initialisation on the device (osar_surface_init): the Elfouhaily
+    spectrum, Elfouhaily spreading and Philox draws of the large workload's
+    2048 x 2048 surface (0.5 m). value = grid points per second."""
+    import torch
+    from oceansar_amd.surface_init import init_surface, init_surface_device
+    kw = dict(Lx=1024., Ly=1024., dx=0.5, dy=0.5, spec_model='elfouhaily',
+              spread_model='elfouhaily', wind_dir=0., wind_fetch=500e3, wind_U=12.,
+              current_mag=0., current_dir=0.)
+    ms = []
+    for i in range(args.warmup + args.steps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        s = init_surface_device(seed=11 + i, **kw)
+        b.record()
+        torch.cuda.synchronize()
+        if i >= args.warmup:
+            ms.append(a.elapsed_time(b))
+    n = s.Nx * s.Ny
+    dt = float(np.mean(ms)) * 1e-3
+    progress('device init done: %.3f ms per surface' % (dt * 1e3))
+    nbytes = n * (8 + 4)           # complex64 coefficients and float32 spectrum written
+    cpu = None
+    if not args.no_cpu:
+        threads, n_os, model = host_cores()
+        t0 = time.perf_counter()
+        init_surface(rng_wave=np.random.default_rng(3), **kw)
+        t_cpu = time.perf_counter() - t0
+        cpu = {'value': n / t_cpu, 'unit': 'points/s', 'cores': 1, 'kind': 'port',
+               'sample': 'the same 2048 x 2048 surface, host numpy init (float32 tables '
+                         'as the reference); %.1f s' % t_cpu,
+               'host': {'os_cpu_count': n_os, 'cpu_model': model}}
+    achieved = nbytes / dt / 1e9
+    return {
+        'metric': 'grid points/s (OceanSurface.init on the device: spectrum, spreading, '
+                  'random coefficients)',
+        'value': n / dt, 'unit': 'points/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': dt * 1e3, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64 (float32 tables out)',
+        'data': 'synthetic (Philox4x32-10 draws)',
+        'config': {'workload': '2048 x 2048 surface at 0.5 m, U10 12 m/s, Elfouhaily spectrum '
+                               'and spreading (host call included: the host computes the '
+                               'grid and scalars, the device the %d points)' % n},
+        'roofline': {'bound': 'valu', 'kernel': 'surface_init_kernel', 'achieved': achieved,
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': None, 'algorithmic_bytes_per_launch': nbytes,
+                     'note': 'FP64 spectrum and spreading per point; 12 B written per point'},
+        'cpu_baseline': cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -406,11 +551,14 @@ def main():
                     help='two streams: synthesis/Bragg of chunk c+1 beside the accumulation '
                          'of chunk c (measured no faster on the large workload: every stage '
                          'already fills the GPU)')
-    ap.add_argument('--form', default='plain', choices=('plain', 'batch', 'serve', 'insar'),
+    ap.add_argument('--form', default='plain',
+                    choices=('plain', 'batch', 'serve', 'insar', 'focus', 'init'),
                     help='plain: one scene (the headline line); batch: 64 mixed scenes, '
                          'plain vs concurrent vs one graph launch; serve: Poisson job '
                          'arrivals, p50/p99 latency vs sequential plain execution; insar: '
-                         'InSAR L1b processing of the large workload\'s 4-channel SLC')
+                         'InSAR L1b processing of the large workload\'s 4-channel SLC; '
+                         'focus: sar_focus of its raw cube; init: OceanSurface.init of its '
+                         'surface on the device')
     ap.add_argument('--jobs', type=int, default=None, help='scenes (batch) or jobs (serve)')
     ap.add_argument('--batch-chunk', type=int, default=32,
                     help='batch form: snapshot intervals per persistent round')
@@ -434,6 +582,12 @@ def main():
         return
     if args.form == 'insar':
         print(json.dumps(bench_insar(args)))
+        return
+    if args.form == 'focus':
+        print(json.dumps(bench_focus(args)))
+        return
+    if args.form == 'init':
+        print(json.dumps(bench_init(args)))
         return
 
     import torch
