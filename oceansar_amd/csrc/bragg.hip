@@ -135,7 +135,11 @@ __device__ __forceinline__ float cabs2f(cf a) { return a.re * a.re + a.im * a.im
 // its RCS sums to rcs_sum[i][pol] (one atomic per workgroup and pol). POL
 // (0 vv, 1 hh, 2 DP) and SPREAD (0 Elfouhaily, 1 cos^2) are compile-time.
 template <bool UNIFORM, int POL, int SPREAD>
-__global__ __launch_bounds__(kThreads) void facet_bragg_kernel(osar_bragg_desc d, BraggConst c,
+// Seven waves per SIMD (at most 72 VGPRs, no spills): the facet loop waits on
+// its slope loads for about 40 % of the time, and one more wave hides more of
+// it than the register budget of six costs (15.3 against 15.5 ms per
+// 512-refresh launch; eight waves need spills and take 17.9 ms).
+__global__ __launch_bounds__(kThreads, 7) void facet_bragg_kernel(osar_bragg_desc d, BraggConst c,
                                                                int cols_per_block) {
   const size_t n = (size_t)d.nx * d.ny;
   constexpr int npol = POL == 2 ? 2 : 1;
