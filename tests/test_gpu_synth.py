@@ -184,6 +184,47 @@ def test_synth_general_row_pass_matches_mirror_pass(need_gpu, tmp_path_factory, 
         assert np.array_equal(res[0][n], res[1][n]), n
 
 
+@pytest.mark.parametrize('lx,times', [(16., [0.3, 2.9, 7.7]), (256., [0.7, 3.1, 5.2, 11.0]),
+                                      (256., [4.4])])
+def test_synth_time_paired_field(need_gpu, tmp_path_factory, lx, times):
+    """The five fields of the Bragg-only run loop: (Dz, Dx) pair up, the slopes
+    share a complex64 spectrum, and Dy pairs with itself at the next snapshot
+    (one transform for snapshots ta and ta + 1; a last odd snapshot alone, a
+    single snapshot without pairing). The general row pass (any wave-number
+    tables) and the mirror-grid pass give the same bits, every field matches
+    the oracle as the unpaired synthesis does, and a second call gives the
+    same bits (a self-paired row stores each column once)."""
+    import torch
+    import oracle.surface as osurf
+    from oceansar_amd import configs
+    from oceansar_amd.engine import DeviceScene, synth_fields
+    sc = configs.load_scene(configs.workload('small', L=lx, dx=1.), tmp_path_factory.mktemp('tp'))
+    ds = DeviceScene(sc)
+    names = ['Dx', 'Dy', 'Dz', 'Diffx', 'Diffy']
+    res = []
+    for mirror in (True, False, True):
+        out = {n: torch.empty((len(times), sc.Nx, sc.Ny), dtype=torch.float32, device='cuda')
+               for n in names}
+        keep = synth_fields(ds.ctx, sc.surface, ds.kx, ds.ky, ds.coefs, times, out, names,
+                            mirror=mirror, slope_fp32=True)
+        torch.cuda.synchronize()
+        del keep
+        res.append({n: out[n].cpu().numpy() for n in names})
+    for n in names:
+        assert np.array_equal(res[0][n], res[1][n]), ('general vs mirror', n)
+        assert np.array_equal(res[0][n], res[2][n]), ('repeat', n)
+    for ti, t in enumerate(times):
+        ref = osurf.synth(sc.surface, t, ('D', 'Diff'))
+        for n in names:
+            got, r = res[0][n][ti].T, ref[n]
+            err = np.max(np.abs(got - r)) / np.sqrt(np.mean(r.astype(np.float64) ** 2))
+            if n in ('Dx', 'Dy', 'Dz'):
+                assert np.mean(got == r) > 0.999, (n, t)
+                assert err < 1e-6, (n, t, err)
+            else:
+                assert err < 5e-7, (n, t, err)
+
+
 def test_end_to_end_matches_oracle(need_gpu, tmp_path_factory):
     """Full GPU loop (synthesis, Bragg, echo) on a pulse window crossing chunks."""
     import torch
