@@ -335,12 +335,12 @@ def workspace_bytes(sc, chunk=32):
     """Synthesis workspace of a context serving scenes of this size: per
     snapshot of a chunk one complex128 plane per packed spectrum (and, for
     Bragg-only scenes, one complex64 plane of the two slopes), and the float64
-    dispersion table (osar_surface_workspace_bytes)."""
+    dispersion and 1/|k| tables (osar_surface_workspace_bytes)."""
     from .engine import scene_fields, spectra_bytes
     n2 = sc.Nx * sc.Ny
     per = spectra_bytes(scene_fields(sc), sc.surface.choppy,
                         sc.bragg_enable and not sc.spec_enable)
-    return int(per * (chunk + 1) * n2 + 8 * n2)
+    return int(per * (chunk + 1) * n2 + 16 * n2)
 
 
 class PersistentBatch:

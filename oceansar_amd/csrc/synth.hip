@@ -97,6 +97,7 @@ struct SynthArgs {
   unsigned char* work;  // workspace: per snapshot, the planes of the nq spectra
   long long snap_bytes; // workspace bytes per snapshot
   double* omega;        // [ny][nx] omega(|k|), filled by omega_kernel
+  double* ikt;          // [ny][nx] 1/|k| (0 off the good-k set), filled by omega_kernel
   QPlan q[kMaxQ];
   // choppy curvatures: the keep spectra A = (a_x, a_xx), B = (a_y, a_yy),
   // C = (Dxx, a_xxx), D = (Dyy, a_yyy), E = (Dxy, a_xxy) (plan indices, -1 = none)
@@ -195,17 +196,23 @@ constexpr int kColRadixLog = 3;                  // radix-8 passes (one butterfl
 constexpr int kWsBlock = 8;                      // workspace run: 8 columns = 128 bytes
 constexpr int kRowPairs = kMaxN / kRowThreads;   // column pairs a row thread owns
 
-// Dispersion relation omega(|k|) over the wave-number grid, once per call: the
-// row pass reads it instead of evaluating sqrt and tanh per snapshot (the same
-// expression, so the same bits).
+// |k| and 1/|k| (0 off the good-k set |k| > thr, as ocean.py:141-148 masks
+// k/|k|) of the wave vector (kx, ky): one expression for every caller, so the
+// same bits.
+__device__ __forceinline__ double knorm(double kx, double ky) { return sqrt(kx * kx + ky * ky); }
+__device__ __forceinline__ double inv_knorm(double k, double thr) { return k > thr ? 1. / k : 0.; }
+
+// Dispersion relation omega(|k|) and 1/|k| over the wave-number grid, once per
+// call: the row pass reads them instead of evaluating sqrt, tanh and a
+// division per snapshot (the same expressions, so the same bits).
 __global__ void omega_kernel(SynthArgs a) {
   const int64_t n = (int64_t)a.nx * a.ny;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int j = (int)(e >> a.logx), i = (int)(e & (a.nx - 1));
-    const double x = a.kx[i], y = a.ky[j];
-    const double k = sqrt(x * x + y * y);
+    const double k = knorm(a.kx[i], a.ky[j]);
     a.omega[e] = a.depth > 0. ? sqrt(kG * k * tanh(k * a.depth)) : sqrt(kG * k);
+    a.ikt[e] = inv_knorm(k, a.thr);
   }
 }
 
@@ -365,12 +372,8 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
         if (kxm == -kx && (double)kyjm == -(double)kyj) {
           // the packed multiplier mA + i mB at -k is conj(mA(k)) + i conj(mB(k))
           const double2 h = H[u];
-          // 1/|k| (0 off the good-k set), recomputed rather than kept: registers
-          double ik = 0.;
-          if (need_ik) {
-            const double k = sqrt(kx * kx + (double)kyj * (double)kyj);
-            ik = k > a.thr ? 1. / k : 0.;
-          }
+          // 1/|k| (0 off the good-k set) from its table: |-k| = |k| here
+          const double ik = need_ik ? a.ikt[(size_t)j * nx + i] : 0.;
           // omega(|k|) = omega(|-k|): from the dispersion table when a multiplier needs it
           const double om = (need_om || hmtf) ? a.omega[(size_t)j * nx + i] : 0.;
           if (need_g) {
@@ -423,19 +426,17 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
         if (i == nx / 2 && j != jm) {
           A0 = nyq[0];
           A1 = nyq[1];
-          k0 = sqrt(kx * kx + ky * ky);
         } else if (MIRROR) {
           // self-paired row: a(-k) is this row's coefficient at column im
           A0 = S[pad(nx + i)];
           A1 = S[pad(nx + im)];
-          k0 = sqrt(kx * kx + ky * ky);
         } else {
           phased_pair<MIRROR>(a, j, i, jm, im, t, A0, A1);
-          k0 = sqrt(kx * kx + ky * ky);
         }
-        const double ik = k0 > a.thr ? 1. / k0 : 0.;
-        const double km = sqrt(kxm * kxm + kym * kym);
-        const double ikm = km > a.thr ? 1. / km : 0.;
+        (void)ky;
+        (void)kym;
+        const double ik = a.ikt[(size_t)j * nx + i];
+        const double ikm = a.ikt[(size_t)jm * nx + im];
         const double om0 = (need_om || hmtf) ? a.omega[(size_t)j * nx + i] : 0.;
         const double omm = (need_om || hmtf) ? a.omega[(size_t)jm * nx + im] : 0.;
         const double yam = mono_y(q.ca, kym), ybm = mono_y(q.cb, kym);
@@ -891,9 +892,9 @@ extern "C" int64_t osar_surface_workspace_bytes(const osar_surface_desc* d) {
   QPlan q[kMaxQ];
   int nq = 0;
   if (make_plan(d, q, &nq) != OSAR_OK) return -1;
-  // packed spectra [n_times][nq planes], then the omega table [ny][nx]
+  // packed spectra [n_times][nq planes], then the omega and 1/|k| tables [ny][nx]
   return (int64_t)plan_layout(d->nx, d->ny, q, nq) * d->n_times +
-         (int64_t)d->nx * d->ny * (int64_t)sizeof(double);
+         2 * (int64_t)d->nx * d->ny * (int64_t)sizeof(double);
 }
 
 namespace {
@@ -963,6 +964,7 @@ int surface_synth(osar_ctx* ctx, const osar_surface_desc* d, void* stream, unsig
   a.snap_bytes = plan_layout(d->nx, d->ny, a.q, a.nq);
   a.work = work;
   a.omega = (double*)(a.work + (size_t)a.snap_bytes * d->n_times);
+  a.ikt = a.omega + (size_t)d->nx * d->ny;
   int rc = twiddles(ctx, d->nx, &a.twx);
   if (rc) return rc;
   rc = twiddles(ctx, d->ny, &a.twy);

@@ -206,7 +206,8 @@ def test_workspace_bytes_match_the_library(tmp_path):
         d.fields = sum(_lib.FIELD_BITS[n] for n in names)
         d.choppy, d.slope_fp32 = int(choppy), int(f32)
         want = lib.osar_surface_workspace_bytes(ctypes.byref(d))
-        assert want == spectra_bytes(names, choppy, f32) * 64 * 32 + 8 * 64 * 32, \
+        # the spectra, then the dispersion and 1/|k| tables
+        assert want == spectra_bytes(names, choppy, f32) * 64 * 32 + 16 * 64 * 32, \
             (names, choppy, f32)
 
 
