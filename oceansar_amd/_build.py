@@ -22,8 +22,13 @@ FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
 # pass: the 1024^2 specular workload's accumulation 60.1 -> 53.2 ms, the plain
 # 4-channel kernel 127 -> 103 VGPRs at the same time); bragg.hip and init.hip
 # need fewer registers without it.
+# bragg.hip: its float32 facet model takes the hardware's float32 division and
+# square root (within about 1 ulp) instead of correctly rounded sequences;
+# the amplitudes stay within 1e-6 of the FP64 model (the same as before) and
+# the facet kernel takes 14.2 instead of 15.3 ms per 512-refresh launch.
 _NO_MLICM = ['-mllvm', '-disable-machine-licm']
-SOURCE_FLAGS = {'synth.hip': _NO_MLICM, 'echo.hip': _NO_MLICM, 'bragg.hip': _NO_MLICM,
+SOURCE_FLAGS = {'synth.hip': _NO_MLICM, 'echo.hip': _NO_MLICM,
+                'bragg.hip': _NO_MLICM + ['-fno-hip-fp32-correctly-rounded-divide-sqrt'],
                 'init.hip': _NO_MLICM}
 
 
