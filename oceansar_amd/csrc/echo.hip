@@ -427,7 +427,10 @@ __device__ __forceinline__ void echo_item(const osar_echo_desc& d, unsigned char
   const float wh = (float)((double)rk / (double)d.K);   // w_h, w_l rounded to float32
   const float wl = (float)(1.0 - (double)wh);
   const size_t plane = (size_t)d.nx * d.ny;
-  const bool bragg = (d.flags & OSAR_ECHO_BRAGG) != 0;
+  // The plain variants (no specular, no cross-track phase) run only with the
+  // Bragg flag set (their launches skip calls without it), so their column
+  // loop has no branch around the Bragg loads.
+  const bool bragg = (SPEC || XTI) ? (d.flags & OSAR_ECHO_BRAGG) != 0 : true;
   int rslot = bragg ? d.rcs_slot[pc] - d.rcs0 : 0;
   const bool bragg_ok = bragg && rslot >= 0 && rslot < d.n_rcs_slots;
   // Amplitude planes by 32-bit byte offsets from the refresh slot in force at
@@ -520,6 +523,11 @@ __device__ __forceinline__ void echo_item(const osar_echo_desc& d, unsigned char
     load_fields<NPOL, SPEC>(B, idx4_lo, f0, f1, cur);
     load_scat<NPOL, SPEC>(d, B, idx4_lo, f0, f1, p, bragg, ia, cur);
     load_index(d, idx4_lo + (uint32_t)(min(x_lo + 1, x_hi - 1) - x_lo) * col4, bragg, ia);
+    // s_waitcnt vmcnt(0): the row's first loads complete here, so that inside
+    // the column loop (which issues its loads in a fixed order, unconditionally
+    // in the plain variants) each wait covers only the loads of the column
+    // before, not the ones just issued for the next columns
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     for (int x = x_lo; x < x_hi; ++x) {
       const uint32_t idx4n = idx4_lo + (uint32_t)(min(x + 1, x_hi - 1) - x_lo) * col4;
       const uint32_t idx4nn = idx4_lo + (uint32_t)(min(x + 2, x_hi - 1) - x_lo) * col4;
@@ -736,6 +744,7 @@ __global__ __launch_bounds__(kThreads, 4) void echo_batch_kernel(
     const int q = p.n_items / osar::kXcds, r = p.n_items % osar::kXcds;
     const int item = x * q + min(x, r) + (c - qs[k]);
     const osar_echo_desc& d = descs[k];
+    if (!(d.flags & OSAR_ECHO_BRAGG)) continue;   // no scattering: nothing to add
     if (NCHF > 0)
       echo_item<P, (NCHF > 0 ? NCHF : 1), NPOL, false, false>(d, lds_raw, item, p.W, p.R, p.LS,
                                                               p.n_pgroups, p.n_xchunks, p.CX);
@@ -780,6 +789,8 @@ int launch_accumulate(osar_ctx* ctx, const osar_echo_desc* d, int W, int R, hipS
   const int n_xchunks = (d->nx + CX - 1) / CX;
   const int64_t n_items = (int64_t)n_pgroups * n_xchunks * ((d->ny + R - 1) / R);
   OSAR_REQUIRE(n_items < INT_MAX, OSAR_EINVAL, "too many workgroups (%lld)", (long long)n_items);
+  // neither Bragg nor specular scattering: the call adds nothing to the lines
+  if (!SPEC && !XTI && !(d->flags & OSAR_ECHO_BRAGG)) return OSAR_OK;
   hipLaunchKernelGGL((echo_accumulate_kernel<P, NCH, NPOL, SPEC, XTI>), dim3((unsigned)n_items),
                      dim3(kThreads), lds, s, *d, W, R, LS, n_pgroups, n_xchunks, CX,
                      (int)n_items);

@@ -49,38 +49,69 @@ __global__ __launch_bounds__(kFocThreads) void focus_rows_kernel(FocusArgs a) {
   const int n = a.nrg, ns = a.nseq_r;
   const int line0 = blockIdx.x * ns;                     // flattened (pol, az) line index
   const int total = ns * n;
-  for (int e = threadIdx.x; e < total; e += kFocThreads) {
-    const int s = e >> a.lognrg, r = e & (n - 1);
-    const int line = line0 + s;
-    const int pol = line / a.naz, az = line - pol * a.naz;
-    double2 v = make_double2(0., 0.);
-    if (FWD) {
-      if (az < a.az0 && r < a.rg0) {
-        const osar_c32 c = a.raw[(size_t)pol * a.raw_pol_stride + (size_t)az * a.rg0 + r];
-        v = make_double2((double)c.re, -(double)c.im);   // conjugate in
-      }
-    } else {
-      v = a.work[(size_t)line * n + r];
+  // Every load of the block is issued before the first is used (unrolled,
+  // unconditional: indices clamped into the block and the raw plane, values
+  // outside selected away afterwards), so the loads overlap instead of each
+  // waiting for the one before.
+  double2 v[kFocEmax];
+  if (FWD) {
+    osar_c32 c[kFocEmax];
+#pragma unroll
+    for (int k = 0; k < kFocEmax; ++k) {
+      const int e = min((int)threadIdx.x + k * kFocThreads, total - 1);
+      const int s = e >> a.lognrg, r = e & (n - 1);
+      const int line = line0 + s;
+      const int pol = line / a.naz, az = line - pol * a.naz;
+      c[k] = a.raw[(size_t)pol * a.raw_pol_stride + (size_t)min(az, a.az0 - 1) * a.rg0 +
+                   min(r, a.rg0 - 1)];
     }
-    buf[pad(e)] = v;
+#pragma unroll
+    for (int k = 0; k < kFocEmax; ++k) {
+      const int e = min((int)threadIdx.x + k * kFocThreads, total - 1);
+      const int s = e >> a.lognrg, r = e & (n - 1);
+      const int line = line0 + s;
+      const int az = line % a.naz;
+      // zero beyond the raw lines and samples; the conjugate in
+      v[k] = (az < a.az0 && r < a.rg0) ? make_double2((double)c[k].re, -(double)c[k].im)
+                                       : make_double2(0., 0.);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kFocEmax; ++k) {
+      const int e = min((int)threadIdx.x + k * kFocThreads, total - 1);
+      v[k] = a.work[(size_t)(line0 + (e >> a.lognrg)) * n + (e & (n - 1))];
+    }
   }
+  // a clamped index writes the block's last element again, with its own value
+#pragma unroll
+  for (int k = 0; k < kFocEmax; ++k)
+    buf[pad(min((int)threadIdx.x + k * kFocThreads, total - 1))] = v[k];
   __syncthreads();
   if (n >= 64)
     lds_ifft_any<kFocThreads, kFocEmax, 3, true>(buf, n, a.lognrg, ns, tw);
   else
     lds_ifft<kFocThreads, kFocEmax, 3, false>(buf, n, a.lognrg, ns, n, tw);
   const double scale = 1.0 / (double)n;
-  for (int e = threadIdx.x; e < total; e += kFocThreads) {
+  double2 hv[kFocEmax];   // the azimuth filter of each element's line, loaded up front
+  if (!FWD) {
+#pragma unroll
+    for (int k = 0; k < kFocEmax; ++k) {
+      const int e = min((int)threadIdx.x + k * kFocThreads, total - 1);
+      hv[k] = a.h[(line0 + (e >> a.lognrg)) % a.naz];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kFocEmax; ++k) {
+    const int e = threadIdx.x + k * kFocThreads;
+    if (e >= total) break;
     const int s = e >> a.lognrg, r = e & (n - 1);
     const int line = line0 + s;
-    double2 v = buf[pad(e)];
-    if (FWD) {
-      v = conj2(v);
-    } else {
-      const int az = line % a.naz;
-      v = cmul2(make_double2(v.x * scale, v.y * scale), a.h[az]);
-    }
-    a.work[(size_t)line * n + r] = v;
+    double2 w = buf[pad(e)];
+    if (FWD)
+      w = conj2(w);
+    else
+      w = cmul2(make_double2(w.x * scale, w.y * scale), hv[k]);
+    a.work[(size_t)line * n + r] = w;
   }
 }
 
@@ -99,10 +130,19 @@ __global__ __launch_bounds__(kFocThreads) void focus_cols_kernel(FocusArgs a) {
   const int c0 = blk * cb;
   double2* plane = a.work + (size_t)pol * a.naz * a.nrg;
   const int total = cb * n;
-  for (int e = threadIdx.x; e < total; e += kFocThreads) {
+  // every load issued before the first is used (see focus_rows_kernel)
+  double2 v[kFocEmax];
+#pragma unroll
+  for (int k = 0; k < kFocEmax; ++k) {
+    const int e = min((int)threadIdx.x + k * kFocThreads, total - 1);
     const int j = e % cb, az = e / cb;               // runs of cb adjacent columns per line
-    const double2 v = plane[(size_t)az * a.nrg + c0 + j];
-    buf[pad(j * n + az)] = FWD ? conj2(v) : v;
+    v[k] = plane[(size_t)az * a.nrg + c0 + j];
+  }
+  // a clamped index writes the block's last element again, with its own value
+#pragma unroll
+  for (int k = 0; k < kFocEmax; ++k) {
+    const int e = min((int)threadIdx.x + k * kFocThreads, total - 1);
+    buf[pad((e % cb) * n + e / cb)] = FWD ? conj2(v[k]) : v[k];
   }
   __syncthreads();
   if (n >= 64)
@@ -111,20 +151,32 @@ __global__ __launch_bounds__(kFocThreads) void focus_cols_kernel(FocusArgs a) {
     lds_ifft<kFocThreads, kFocEmax, 3, false>(buf, n, a.lognaz, cb, n, tw);
   const double scale = 1.0 / (double)n;
   const int nkeep = a.az_hi - a.az_lo;
-  for (int e = threadIdx.x; e < total; e += kFocThreads) {
+  double rc[kFocEmax], fq[kFocEmax];   // RCMC terms of each element, loaded up front
+  if (FWD) {
+#pragma unroll
+    for (int k = 0; k < kFocEmax; ++k) {
+      const int e = min((int)threadIdx.x + k * kFocThreads, total - 1);
+      rc[k] = a.rcmc[e / cb];
+      fq[k] = a.fr[c0 + e % cb];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kFocEmax; ++k) {
+    const int e = threadIdx.x + k * kFocThreads;
+    if (e >= total) break;
     const int j = e % cb, az = e / cb;
     const int r = c0 + j;
-    double2 v = buf[pad(j * n + az)];
+    double2 w = buf[pad(j * n + az)];
     if (FWD) {
-      v = conj2(v);
+      w = conj2(w);
       // numpy's order: exp(((4 pi i) rcmc / c) fr), a purely imaginary argument
-      const double t = (4. * M_PI * a.rcmc[az]) / 299792458.0;
+      const double t = (4. * M_PI * rc[k]) / 299792458.0;
       double sn, cs;
-      sincos(t * a.fr[r], &sn, &cs);
-      plane[(size_t)az * a.nrg + r] = cmul2(v, make_double2(cs, sn));
+      sincos(t * fq[k], &sn, &cs);
+      plane[(size_t)az * a.nrg + r] = cmul2(w, make_double2(cs, sn));
     } else if (az >= a.az_lo && az < a.az_hi && r < a.rg0) {
       a.slc[((size_t)pol * nkeep + (az - a.az_lo)) * a.rg0 + r] =
-          make_double2(v.x * scale, v.y * scale);
+          make_double2(w.x * scale, w.y * scale);
     }
   }
 }

@@ -519,28 +519,36 @@ __global__ __launch_bounds__(kColThreads, kColWaves) void synth_cols_kernel(Synt
   const int cbw = q.cbw;
   const size_t wofs = (size_t)(x0 >> q.lcbw) * ny * cbw + (x0 & (cbw - 1));
   const unsigned char* wp = a.work + (size_t)ti * (size_t)a.snap_bytes + (size_t)q.off;
-  // every load of the block in flight before the first LDS store
+  // every load of the block in flight before the first LDS store: the loads
+  // are unconditional (indices past the block clamped to its last element)
+  // and complex64 elements are widened after all of them are issued, so no
+  // load waits for the one before it
   constexpr int kFill = kColElems / kColThreads;
+  const int nblk = ny * cb;
   double2 fill[kFill];
   if (q.f32) {
     const float2* w = (const float2*)wp + wofs;
+    nt_float2 v32[kFill];
 #pragma unroll
     for (int k = 0; k < kFill; ++k) {
-      const int e = tid + k * kColThreads;
-      if (e < ny * cb) fill[k] = load_stream32(&w[(size_t)(e >> a.lcb) * cbw + (e & (cb - 1))]);
+      const int e = min(tid + k * kColThreads, nblk - 1);
+      v32[k] = __builtin_nontemporal_load(
+          (const nt_float2*)&w[(size_t)(e >> a.lcb) * cbw + (e & (cb - 1))]);
     }
+#pragma unroll
+    for (int k = 0; k < kFill; ++k) fill[k] = make_double2((double)v32[k].x, (double)v32[k].y);
   } else {
     const double2* w = (const double2*)wp + wofs;
 #pragma unroll
     for (int k = 0; k < kFill; ++k) {
-      const int e = tid + k * kColThreads;
-      if (e < ny * cb) fill[k] = load_stream(&w[(size_t)(e >> a.lcb) * cbw + (e & (cb - 1))]);
+      const int e = min(tid + k * kColThreads, nblk - 1);
+      fill[k] = load_stream(&w[(size_t)(e >> a.lcb) * cbw + (e & (cb - 1))]);
     }
   }
 #pragma unroll
   for (int k = 0; k < kFill; ++k) {
     const int e = tid + k * kColThreads;
-    if (e < ny * cb) C[pad((e & (cb - 1)) * ny + (e >> a.lcb))] = fill[k];
+    if (e < nblk) C[pad((e & (cb - 1)) * ny + (e >> a.lcb))] = fill[k];
   }
   __syncthreads();
   lds_ifft_any<kColThreads, kColElems / kColThreads, kColRadixLog, SHORT>(C, ny, a.logy, cb, tw);

@@ -146,6 +146,31 @@ def test_accumulator_launch_shapes_agree(need_gpu, cfg1_scene):
         assert rel_rms(o, outs[0]) < 2e-6
 
 
+def test_accumulator_without_scattering_adds_nothing(need_gpu, cfg1_scene):
+    """Neither Bragg nor specular scattering: the call adds nothing (the plain
+    accumulator, which assumes the Bragg flag, is not launched) and the lines
+    keep what they held."""
+    import torch
+    from oceansar_amd.engine import DeviceScene
+    sc = cfg1_scene
+    p0, n = 500, 24
+    m0, m1 = p0 // sc.K, (p0 + n - 1) // sc.K + 1
+    _, dev = _snapshots(sc, m0, m1)
+    ds = DeviceScene(sc)
+    raw = ds.new_raw()
+    raw.real.fill_(1.5)
+    raw.imag.fill_(-0.25)
+    before = raw.clone()
+    saved = sc.bragg_enable
+    try:
+        sc.bragg_enable = False
+        ds.accumulate(dev['Dx'], dev['Dy'], dev['Dz'], m0, None, 0, p0, n, raw)
+    finally:
+        sc.bragg_enable = saved
+    torch.cuda.synchronize()
+    assert torch.equal(raw, before)
+
+
 def test_accumulator_rejects_scatterer_series_beyond_32bit_offsets(need_gpu, cfg1_scene):
     """The scatterer gathers use 32-bit byte offsets: a series of more than 2^29
     entries, or one too short for the pulses, is refused before any launch."""
