@@ -164,11 +164,21 @@ __device__ __forceinline__ void load_twiddles(double2* dst, const double2* src, 
 // SHORT: every pass of the transform has N/R and Ns (after the first) multiples
 // of 8, which holds for N >= 64 with radix 8/4 passes; otherwise (N = 16, 32)
 // every address is padded on its own.
+// The thread's index, opaque to the optimiser: every pass recomputes the
+// index arithmetic derived from it instead of values hoisted out of the
+// callers' loops staying live in registers (across a persistent kernel's
+// work items they pushed the column pass of the synthesis into spills).
+__device__ __forceinline__ int pass_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 template <int NT, int R, int BPT, bool SHORT>
 __device__ __forceinline__ void stockham_pass(double2* buf, int logN, int Ns, int logNs, int nseq,
                                               int ss, const double2* tw) {
   constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
-  const int tid = threadIdx.x;
+  const int tid = pass_tid();
   const int lnb = logN - LR;
   const int nb = 1 << lnb;
   const int total = nseq << lnb;
@@ -282,7 +292,7 @@ __device__ __forceinline__ void stockham_pass_n(double2* buf, int nseq, const do
   static_assert(nb % 8 == 0 && (Ns == 1 || Ns % 8 == 0), "padded strides need multiples of 8");
   constexpr int rd_step = nb + (nb >> 3);
   constexpr int wr_step = Ns + (Ns >> 3);
-  const int tid = threadIdx.x;
+  const int tid = pass_tid();
   const int total = nseq << lnb;
   double2 v[BPT][R];
 #pragma unroll

@@ -360,22 +360,32 @@ __global__ __launch_bounds__(kRowThreads, kRowWaves) void synth_rows_kernel(Synt
       int logx = a.logx, tq = tid;
       asm volatile("" : "+s"(logx), "+v"(tq));
       unsigned general = 0;   // column pairs that need the general form
+      // the table values of every column pair, all loaded before the first is
+      // used; re-read per spectrum at the opaque index, so the multipliers
+      // derived from them are recomputed rather than held across spectra
+      double kxv[kRowPairs], kxmv[kRowPairs], ikv[kRowPairs], omv[kRowPairs];
+#pragma unroll
+      for (int u = 0; u < kRowPairs; ++u) {
+        const int i = min(tq + u * kRowThreads, nx - 1);
+        const int im = (nx - i) & (nx - 1);
+        kxv[u] = a.kx[i];
+        kxmv[u] = a.kx[im];
+        ikv[u] = need_ik ? a.ikt[(size_t)j * nx + i] : 0.;
+        omv[u] = (need_om || hmtf) ? a.omega[(size_t)j * nx + i] : 0.;
+      }
 #pragma unroll
       for (int u = 0; u < kRowPairs; ++u) {
         const bool in = tq + u * kRowThreads < nx;
         const int i = min(tq + u * kRowThreads, nx - 1);
         const int im = (nx - i) & (nx - 1);
-        // kx is re-read from its (cached) table at the opaque index, so the
-        // multipliers derived from it are recomputed here rather than hoisted
-        // (one set per multiplier kind).
-        const double kx = a.kx[i], kxm = a.kx[im];
+        const double kx = kxv[u], kxm = kxmv[u];
         if (kxm == -kx && (double)kyjm == -(double)kyj) {
           // the packed multiplier mA + i mB at -k is conj(mA(k)) + i conj(mB(k))
           const double2 h = H[u];
           // 1/|k| (0 off the good-k set) from its table: |-k| = |k| here
-          const double ik = need_ik ? a.ikt[(size_t)j * nx + i] : 0.;
+          const double ik = ikv[u];
           // omega(|k|) = omega(|-k|): from the dispersion table when a multiplier needs it
-          const double om = (need_om || hmtf) ? a.omega[(size_t)j * nx + i] : 0.;
+          const double om = omv[u];
           if (need_g) {
             // G(a) from a(k) and a(-k), recomputed; the thread of the mirror
             // column of a self-paired row computes -conj(G) to the bit
