@@ -31,7 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 (vector, packed) peak
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak (published)
 HBM_PEAK_GBS = 8000.0
 _T0 = time.perf_counter()
 
